@@ -1,0 +1,162 @@
+/*
+ * splash.h -- C ABI of the MI355X-native SPLASH MUSCL time step.
+ *
+ * The reference (TURBOLOSE/MUSCL-scheme-on-spherical-mesh-WIP) has no FFI:
+ * its boundary is the C++ class API MUSCL_HLLE / MUSCL_HLLC / MUSCL_HLLCplus
+ * (+ MUSCL_HLLE_p) with the methods do_step / time / get_stop_check.  Each
+ * entry point below names the reference member it replaces; the C++ classes
+ * with the reference signatures live in include/splash/MUSCL.hpp and call
+ * only this ABI.
+ *
+ * Conventions: every function returns 0 (SPLASH_OK) or a negative error
+ * code and never throws; the library owns device memory, the caller owns
+ * every host array it passes; one context may be driven by one host thread
+ * at a time.  State arrays are [n_faces][dim] row-major FP64 in the mesh's
+ * face order with slot 4 holding the total energy (dim 5).
+ */
+#ifndef SPLASH_H
+#define SPLASH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPLASH_OK 0
+#define SPLASH_EINVAL (-1)   /* bad argument */
+#define SPLASH_EMESH (-2)    /* mesh is not a closed manifold the stencil search accepts */
+#define SPLASH_EDEVICE (-3)  /* HIP runtime error */
+#define SPLASH_ENOMEM (-4)
+#define SPLASH_EINTERNAL (-5)
+
+/* solver ids: MUSCL_HLLE (HLLE.hpp:6), MUSCL_HLLE_p (HLLE_p.hpp:6),
+ * MUSCL_HLLC (HLLC.hpp:6), MUSCL_HLLCplus (HLLCplus.hpp:6) */
+#define SPLASH_HLLE 0
+#define SPLASH_HLLE_P 1
+#define SPLASH_HLLC 2
+#define SPLASH_HLLCPLUS 3
+
+/* precision modes: plain FP64, or FP32 reconstruction/flux with FP64 state */
+#define SPLASH_FP64 0
+#define SPLASH_MIXED 1
+
+/* stop_check causes (bit set in splash_stop_bits) */
+#define SPLASH_STOP_NAN_FLUX 1        /* MUSCL_base.hpp:264-268 */
+#define SPLASH_STOP_NAN_WAVESPEED 2   /* adiabatic.hpp:671-676 */
+#define SPLASH_STOP_WAVE_BRANCH 4     /* HLLC.hpp:114-119, HLLCplus.hpp:169-174, HLLE.hpp:51-55 */
+#define SPLASH_STOP_NEGATIVE_DT 8     /* adiabatic.hpp:573-577 */
+
+/* Flattened polygon mesh (what MUSCL_geometry.hpp:66-88 extracts from the
+ * pmp::SurfaceMesh): vertex v at vertices[3v..3v+2]; face f has vertices
+ * face_vertices[face_offsets[f] .. face_offsets[f+1]-1] in circulator order. */
+typedef struct splash_mesh_desc {
+    int32_t n_vertices;
+    int32_t n_faces;
+    const double *vertices;
+    const int32_t *face_offsets;
+    const int32_t *face_vertices;
+} splash_mesh_desc;
+
+/* Physics switches read by the adiabatic constructor from
+ * input/parameters.json (adiabatic.hpp:59-68) plus accretion_on (ctor arg). */
+typedef struct splash_params {
+    int32_t accretion_on;
+    int32_t friction_on;
+    double acc_rate;
+    double e_acc;
+    double omega_acc_abs;
+    double tilt_angle;
+    double acc_width;
+    double alpha;
+    int32_t precision; /* SPLASH_FP64 or SPLASH_MIXED */
+    int32_t setup_threads; /* host threads for the geometry precompute, 0 = 1 */
+} splash_params;
+
+typedef struct splash_mesh splash_mesh;   /* library-owned generated mesh */
+typedef struct splash_ctx splash_ctx;     /* one resident run */
+
+/* Fills *p with the reference's shipped input/parameters.json values,
+ * accretion off, FP64. */
+void splash_default_params(splash_params *p);
+
+/* Mesh generators -- sph_gen.h uv_sphere :215, icosphere :267,
+ * icosphere_hex :278, quad_sphere :286; numbering identical to pmp's. */
+int splash_mesh_uv_sphere(int n_slices, int n_stacks, splash_mesh **out);
+int splash_mesh_icosphere(int n_subdivisions, splash_mesh **out);
+int splash_mesh_icosphere_hex(int n_subdivisions, splash_mesh **out);
+int splash_mesh_quad_sphere(int n_subdivisions, splash_mesh **out);
+/* View of a generated mesh; pointers stay valid until splash_mesh_free. */
+int splash_mesh_view(const splash_mesh *m, splash_mesh_desc *out);
+void splash_mesh_free(splash_mesh *m);
+
+/* Replaces the constructors MUSCL_HLLE(mesh, U_in, dim, gam, threads)
+ * (HLLE.hpp:13-14), MUSCL_HLLE_p / MUSCL_HLLC / MUSCL_HLLCplus(mesh, U_in,
+ * dim, gam, omega_ns, accretion_on, threads) (HLLE_p.hpp:11-13,
+ * HLLC.hpp:11-13, HLLCplus.hpp:11-13).  Mesh and U_in are copied.
+ * params may be NULL (defaults).  Returns SPLASH_EINVAL when dim does not
+ * match the solver (4 for HLLE, 5 otherwise); the reference only prints
+ * "check dim" and sets stop_check (adiabatic.hpp:26-30). */
+int splash_create(const splash_mesh_desc *mesh, const double *U_in, int dim, int solver, double gam,
+                  double omega_ns, const splash_params *params, splash_ctx **out);
+void splash_destroy(splash_ctx *ctx);
+
+/* MUSCL_base::do_step(dt0) (MUSCL_base.hpp:97-225).  Enqueues the step on
+ * the context's stream and returns; dt never leaves the GPU. */
+int splash_do_step(splash_ctx *ctx, double dt0);
+/* n consecutive do_step(dt0) calls, replayed from a captured HIP graph. */
+int splash_do_steps(splash_ctx *ctx, double dt0, int n);
+/* Waits for every enqueued step. */
+int splash_synchronize(splash_ctx *ctx);
+
+/* MUSCL_base::time() (MUSCL_base.hpp:227-230); synchronises. */
+int splash_time(splash_ctx *ctx, double *t);
+/* The dt the last step used (the member dt after the clamps). */
+int splash_last_dt(splash_ctx *ctx, double *dt);
+/* MUSCL_base::get_stop_check() (MUSCL_base.hpp:232-235): *stop = 0/1. */
+int splash_stop_check(splash_ctx *ctx, int *stop);
+/* Which causes set stop_check, and the first offending half-edge/face
+ * (code << 40 | index), or ~0 when none. */
+int splash_stop_bits(splash_ctx *ctx, int *bits, uint64_t *first);
+/* Extension (the reference keeps U private): copies the state out. */
+int splash_get_state(splash_ctx *ctx, double *U_out);
+/* The do_step diagnostics sums (MUSCL_base.hpp:187-205):
+ * out = {sum Sigma*A, sum Lx*A, sum Ly*A, sum Lz*A, sum E*A}. */
+int splash_diagnostics(splash_ctx *ctx, double out[5]);
+int splash_num_faces(const splash_ctx *ctx, int32_t *n_faces, int32_t *n_halfedges);
+/* Face centres (MUSCL_geometry.hpp:176-187) and areas (:989-1030). */
+int splash_face_geometry(const splash_ctx *ctx, double *centers_out, double *areas_out);
+
+/* Host geometry tables for checks (MUSCL_geometry.hpp:43-575).  Arrays are
+ * per face [nf] / per half-edge [nh] (h = face_offsets[f] + j); see
+ * splash_geometry_view for the layout. */
+typedef struct splash_geometry splash_geometry;
+typedef struct splash_geometry_view {
+    int32_t nf, nh;
+    const double *fc;          /* [nf][3] */
+    const double *normal;      /* [nf][3] */
+    const double *area;        /* [nf] */
+    const int32_t *nb_edge;    /* [nh] */
+    const int32_t *twin;       /* [nh] */
+    const int32_t *ff_plus;    /* [nh][2] */
+    const int32_t *ff_minus;   /* [nh][2] */
+    const double *beta_plus;   /* [nh][2] */
+    const double *beta_minus;  /* [nh][2] */
+    const double *h_plus;      /* [nh] */
+    const double *h_minus;     /* [nh] */
+    const double *bm_dist;     /* [nh] */
+    const double *en;          /* [nh][3] */
+    double h0;
+    double round_diff;
+} splash_geometry_view;
+int splash_geometry_create(const splash_mesh_desc *mesh, int threads, splash_geometry **out);
+int splash_geometry_get_view(const splash_geometry *g, splash_geometry_view *out);
+void splash_geometry_free(splash_geometry *g);
+
+/* Human-readable text of the last error on this host thread. */
+const char *splash_last_error_message(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPLASH_H */

@@ -1,0 +1,286 @@
+// extern "C" boundary (include/splash.h): converts exceptions to error codes.
+#include "../../include/splash.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+#include "geometry.hpp"
+#include "mesh.hpp"
+
+struct splash_mesh {
+    splash::PolyMesh mesh;
+};
+struct splash_ctx {
+    std::unique_ptr<splash::Engine> engine;
+};
+struct splash_geometry {
+    splash::Geometry geo;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const splash::HipError& e) {
+        return fail(SPLASH_EDEVICE, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(SPLASH_ENOMEM, "out of host memory");
+    } catch (const std::invalid_argument& e) {
+        return fail(SPLASH_EINVAL, e.what());
+    } catch (const std::runtime_error& e) {
+        return fail(SPLASH_EMESH, e.what());
+    } catch (const std::exception& e) {
+        return fail(SPLASH_EINTERNAL, e.what());
+    } catch (...) {
+        return fail(SPLASH_EINTERNAL, "unknown error");
+    }
+}
+
+splash::PolyMesh from_desc(const splash_mesh_desc* d) {
+    if (!d || d->n_vertices <= 0 || d->n_faces <= 0 || !d->vertices || !d->face_offsets || !d->face_vertices)
+        throw std::invalid_argument("invalid mesh descriptor");
+    splash::PolyMesh m;
+    m.xyz.assign(d->vertices, d->vertices + 3 * size_t(d->n_vertices));
+    m.foff.assign(d->face_offsets, d->face_offsets + d->n_faces + 1);
+    if (m.foff[0] != 0) throw std::invalid_argument("face_offsets[0] must be 0");
+    for (int32_t f = 0; f < d->n_faces; ++f)
+        if (m.foff[f + 1] - m.foff[f] < 3) throw std::invalid_argument("every face needs at least 3 vertices");
+    m.fv.assign(d->face_vertices, d->face_vertices + m.foff.back());
+    for (int32_t v : m.fv)
+        if (v < 0 || v >= d->n_vertices) throw std::invalid_argument("face vertex index out of range");
+    return m;
+}
+
+int make_mesh(splash::PolyMesh (*gen)(int), int n, splash_mesh** out) {
+    if (!out) return fail(SPLASH_EINVAL, "out is null");
+    return guarded([&] {
+        auto m = std::make_unique<splash_mesh>();
+        m->mesh = gen(n);
+        *out = m.release();
+        return SPLASH_OK;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+void splash_default_params(splash_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    splash::RunParams d;
+    p->accretion_on = 0;
+    p->friction_on = d.friction_on;
+    p->acc_rate = d.acc_rate;
+    p->e_acc = d.e_acc;
+    p->omega_acc_abs = d.omega_acc_abs;
+    p->tilt_angle = d.tilt_angle;
+    p->acc_width = d.acc_width;
+    p->alpha = d.alpha;
+    p->precision = SPLASH_FP64;
+    p->setup_threads = 0;
+}
+
+int splash_mesh_uv_sphere(int n_slices, int n_stacks, splash_mesh** out) {
+    if (!out) return fail(SPLASH_EINVAL, "out is null");
+    return guarded([&] {
+        auto m = std::make_unique<splash_mesh>();
+        m->mesh = splash::uv_sphere(n_slices, n_stacks);
+        *out = m.release();
+        return SPLASH_OK;
+    });
+}
+int splash_mesh_icosphere(int n, splash_mesh** out) { return make_mesh(splash::icosphere, n, out); }
+int splash_mesh_icosphere_hex(int n, splash_mesh** out) { return make_mesh(splash::icosphere_hex, n, out); }
+int splash_mesh_quad_sphere(int n, splash_mesh** out) { return make_mesh(splash::quad_sphere, n, out); }
+
+int splash_mesh_view(const splash_mesh* m, splash_mesh_desc* out) {
+    if (!m || !out) return fail(SPLASH_EINVAL, "null argument");
+    out->n_vertices = m->mesh.n_vertices();
+    out->n_faces = m->mesh.n_faces();
+    out->vertices = m->mesh.xyz.data();
+    out->face_offsets = m->mesh.foff.data();
+    out->face_vertices = m->mesh.fv.data();
+    return SPLASH_OK;
+}
+
+void splash_mesh_free(splash_mesh* m) { delete m; }
+
+int splash_create(const splash_mesh_desc* mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
+                  const splash_params* params, splash_ctx** out) {
+    if (!out) return fail(SPLASH_EINVAL, "out is null");
+    *out = nullptr;
+    return guarded([&] {
+        splash_params p;
+        if (params) p = *params;
+        else splash_default_params(&p);
+        splash::RunParams rp;
+        rp.accretion_on = p.accretion_on;
+        rp.friction_on = p.friction_on;
+        rp.acc_rate = p.acc_rate;
+        rp.e_acc = p.e_acc;
+        rp.omega_acc_abs = p.omega_acc_abs;
+        rp.tilt_angle = p.tilt_angle;
+        rp.acc_width = p.acc_width;
+        rp.alpha = p.alpha;
+        splash::PolyMesh m = from_desc(mesh);
+        auto ctx = std::make_unique<splash_ctx>();
+        ctx->engine = std::make_unique<splash::Engine>(m, U_in, dim, solver, gam, omega_ns, rp, p.precision,
+                                                       p.setup_threads);
+        *out = ctx.release();
+        return SPLASH_OK;
+    });
+}
+
+void splash_destroy(splash_ctx* ctx) { delete ctx; }
+
+#define CTX_OR_FAIL                                                    \
+    if (!ctx || !ctx->engine) return fail(SPLASH_EINVAL, "null context")
+
+int splash_do_step(splash_ctx* ctx, double dt0) {
+    CTX_OR_FAIL;
+    return guarded([&] {
+        ctx->engine->step(dt0);
+        return SPLASH_OK;
+    });
+}
+
+int splash_do_steps(splash_ctx* ctx, double dt0, int n) {
+    CTX_OR_FAIL;
+    if (n < 0) return fail(SPLASH_EINVAL, "negative step count");
+    return guarded([&] {
+        ctx->engine->steps(dt0, n);
+        return SPLASH_OK;
+    });
+}
+
+int splash_synchronize(splash_ctx* ctx) {
+    CTX_OR_FAIL;
+    return guarded([&] {
+        ctx->engine->synchronize();
+        return SPLASH_OK;
+    });
+}
+
+int splash_time(splash_ctx* ctx, double* t) {
+    CTX_OR_FAIL;
+    if (!t) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *t = ctx->engine->time();
+        return SPLASH_OK;
+    });
+}
+
+int splash_last_dt(splash_ctx* ctx, double* dt) {
+    CTX_OR_FAIL;
+    if (!dt) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *dt = ctx->engine->last_dt();
+        return SPLASH_OK;
+    });
+}
+
+int splash_stop_check(splash_ctx* ctx, int* stop) {
+    CTX_OR_FAIL;
+    if (!stop) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *stop = ctx->engine->stop_bits() != 0;
+        return SPLASH_OK;
+    });
+}
+
+int splash_stop_bits(splash_ctx* ctx, int* bits, uint64_t* first) {
+    CTX_OR_FAIL;
+    return guarded([&] {
+        if (bits) *bits = ctx->engine->stop_bits();
+        if (first) *first = ctx->engine->first_error();
+        return SPLASH_OK;
+    });
+}
+
+int splash_get_state(splash_ctx* ctx, double* U_out) {
+    CTX_OR_FAIL;
+    if (!U_out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        ctx->engine->get_state(U_out);
+        return SPLASH_OK;
+    });
+}
+
+int splash_diagnostics(splash_ctx* ctx, double out[5]) {
+    CTX_OR_FAIL;
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        ctx->engine->diagnostics(out);
+        return SPLASH_OK;
+    });
+}
+
+int splash_num_faces(const splash_ctx* ctx, int32_t* n_faces, int32_t* n_halfedges) {
+    CTX_OR_FAIL;
+    if (n_faces) *n_faces = ctx->engine->n_faces();
+    if (n_halfedges) *n_halfedges = ctx->engine->n_halfedges();
+    return SPLASH_OK;
+}
+
+int splash_face_geometry(const splash_ctx* ctx, double* centers_out, double* areas_out) {
+    CTX_OR_FAIL;
+    const splash::Geometry& g = ctx->engine->geometry();
+    if (centers_out) std::memcpy(centers_out, g.fc.data(), sizeof(double) * g.fc.size());
+    if (areas_out) std::memcpy(areas_out, g.area.data(), sizeof(double) * g.area.size());
+    return SPLASH_OK;
+}
+
+int splash_geometry_create(const splash_mesh_desc* mesh, int threads, splash_geometry** out) {
+    if (!out) return fail(SPLASH_EINVAL, "out is null");
+    *out = nullptr;
+    return guarded([&] {
+        auto g = std::make_unique<splash_geometry>();
+        g->geo = splash::build_geometry(from_desc(mesh), threads);
+        *out = g.release();
+        return SPLASH_OK;
+    });
+}
+
+int splash_geometry_get_view(const splash_geometry* g, splash_geometry_view* v) {
+    if (!g || !v) return fail(SPLASH_EINVAL, "null argument");
+    const splash::Geometry& G = g->geo;
+    v->nf = G.nf;
+    v->nh = G.nh;
+    v->fc = G.fc.data();
+    v->normal = G.normal.data();
+    v->area = G.area.data();
+    v->nb_edge = G.nb_edge.data();
+    v->twin = G.twin.data();
+    v->ff_plus = G.ff_plus.data();
+    v->ff_minus = G.ff_minus.data();
+    v->beta_plus = G.beta_plus.data();
+    v->beta_minus = G.beta_minus.data();
+    v->h_plus = G.h_plus.data();
+    v->h_minus = G.h_minus.data();
+    v->bm_dist = G.bm_dist.data();
+    v->en = G.en.data();
+    v->h0 = G.h0;
+    v->round_diff = G.round_diff;
+    return SPLASH_OK;
+}
+
+void splash_geometry_free(splash_geometry* g) { delete g; }
+
+const char* splash_last_error_message(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+// Host side of one resident SPLASH run: uploads the stencil tables, keeps
+// the state on the GPU and enqueues the per-step kernel sequence.
+#include "engine.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace splash {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+unsigned long long host_dkey(double d) {
+    unsigned long long b;
+    std::memcpy(&b, &d, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+unsigned long long bits(double d) {
+    unsigned long long b;
+    std::memcpy(&b, &d, 8);
+    return b;
+}
+
+inline double norm3(double x, double y, double z) {
+    double s = 0.0;
+    s = s + x * x;
+    s = s + y * y;
+    s = s + z * z;
+    return std::sqrt(s);
+}
+
+}  // namespace
+
+Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
+               const RunParams& params, int precision, int geometry_threads)
+    : dim_(dim), solver_(solver), precision_(precision), gam_(gam), omega_ns_(omega_ns), params_(params) {
+    if (solver < kHLLE || solver > kHLLCPLUS) throw std::invalid_argument("unknown solver");
+    if ((solver == kHLLE && dim != 4) || (solver != kHLLE && dim != 5))
+        throw std::invalid_argument("dim must be 4 for MUSCL_HLLE and 5 for the adiabatic solvers");
+    if (precision != kFP64 && precision != kMixed) throw std::invalid_argument("unknown precision mode");
+    if (!U_in) throw std::invalid_argument("U_in is null");
+    geo_ = build_geometry(mesh, geometry_threads);
+
+    const int32_t nf = geo_.nf;
+    if (dim == 5) {
+        // adiabatic.hpp:70-90 (the index of the :75 loop starts at face 0)
+        double area_coeff = 0;
+        const double mu = M_PI / 2;
+        const double sigma = params.acc_width / 2.355;
+        for (int32_t f = 0; f < nf; f++) {
+            const double* c = &geo_.fc[3 * size_t(f)];
+            const double theta = std::acos(c[2] / norm3(c[0], c[1], c[2]));
+            area_coeff += geo_.area[f] / std::sqrt(2 * M_PI * sigma * sigma) *
+                          std::exp(-(theta - mu) * (theta - mu) / (2 * sigma * sigma));
+        }
+        area_coeff /= 4 * M_PI;
+        area_coeff_ = area_coeff;
+        double tm = 0;
+        for (int32_t f = 0; f < nf; f++) tm += U_in[size_t(f) * dim] * geo_.area[f];
+        total_mass_ = tm;
+    }
+
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+        upload();
+        std::vector<double> U(U_in, U_in + size_t(nf) * dim);
+        check(hipMemcpyAsync(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "upload state");
+        fp64::enqueue_prepare(args_, stream_);
+        check(hipGetLastError(), "prepare launch");
+        check(hipStreamSynchronize(stream_), "prepare");
+    } catch (...) {
+        for (void* p : allocations_) (void)hipFree(p);
+        (void)hipStreamDestroy(stream_);
+        throw;
+    }
+}
+
+Engine::~Engine() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocations_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::upload() {
+    const Geometry& g = geo_;
+    const int32_t nf = g.nf, nh = g.nh;
+    auto dev = [&](size_t bytes, const void* host) -> void* {
+        void* p = nullptr;
+        check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
+        allocations_.push_back(p);
+        if (host) check(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice), "upload");
+        return p;
+    };
+
+    std::vector<int32_t> sten(4 * size_t(nh));
+    std::vector<double> beta(4 * size_t(nh)), ec(4 * size_t(nh)), en(4 * size_t(nh));
+    for (int32_t h = 0; h < nh; ++h) {
+        sten[4 * size_t(h) + 0] = g.ff_plus[2 * size_t(h)];
+        sten[4 * size_t(h) + 1] = g.ff_plus[2 * size_t(h) + 1];
+        sten[4 * size_t(h) + 2] = g.ff_minus[2 * size_t(h)];
+        sten[4 * size_t(h) + 3] = g.ff_minus[2 * size_t(h) + 1];
+        beta[4 * size_t(h) + 0] = g.beta_plus[2 * size_t(h)];
+        beta[4 * size_t(h) + 1] = g.beta_plus[2 * size_t(h) + 1];
+        beta[4 * size_t(h) + 2] = g.beta_minus[2 * size_t(h)];
+        beta[4 * size_t(h) + 3] = g.beta_minus[2 * size_t(h) + 1];
+        for (int c = 0; c < 3; ++c) {
+            ec[4 * size_t(h) + c] = g.ec[3 * size_t(h) + c];
+            en[4 * size_t(h) + c] = g.en[3 * size_t(h) + c];
+        }
+        ec[4 * size_t(h) + 3] = g.ecn[h];
+        en[4 * size_t(h) + 3] = g.chord[h];
+    }
+    std::vector<double> fcn(4 * size_t(nf), 0.0);
+    for (int32_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) fcn[4 * size_t(f) + c] = g.fcn[3 * size_t(f) + c];
+
+    // accretion profile per face (adiabatic.hpp:425-449), evaluated once with
+    // the expressions the reference evaluates on every source() call
+    std::vector<double> acc(8 * size_t(nf), 0.0);
+    if (dim_ == 5 && params_.accretion_on) {
+        const double mu = M_PI / 2;
+        const double sigma = params_.acc_width / 2.355;
+        const double tilt = params_.tilt_angle;
+        for (int32_t f = 0; f < nf; ++f) {
+            const double* r = &fcn[4 * size_t(f)];
+            const double theta_acc = std::acos(r[1] * std::sin(tilt) + r[2] * std::cos(tilt));
+            const double rate = 1 / area_coeff_ * params_.acc_rate / std::sqrt(2 * M_PI * sigma * sigma) *
+                                std::exp(-(theta_acc - mu) * (theta_acc - mu) / (2 * sigma * sigma));
+            const double ox = 0, oy = std::sin(tilt) * params_.omega_acc_abs, oz = std::cos(tilt) * params_.omega_acc_abs;
+            // omxr = omega_acc x r ; rxv = r x omxr
+            const double mx = oy * r[2] - oz * r[1], my = oz * r[0] - ox * r[2], mz = ox * r[1] - oy * r[0];
+            double* a = &acc[8 * size_t(f)];
+            a[0] = rate;
+            a[1] = mx;
+            a[2] = my;
+            a[3] = mz;
+            a[4] = r[1] * mz - r[2] * my;
+            a[5] = r[2] * mx - r[0] * mz;
+            a[6] = r[0] * my - r[1] * mx;
+        }
+    }
+
+    // flux/update blocks: consecutive faces whose half-edges fit one block
+    const int B = fp64::block_size();
+    std::vector<int32_t> blk{0};
+    for (int32_t f = 0; f < nf; ++f) {
+        if (g.foff[f + 1] - g.foff[blk.back()] > B) blk.push_back(f);
+        if (g.foff[f + 1] - g.foff[f] > B) throw std::runtime_error("face valence exceeds the block size");
+    }
+    blk.push_back(nf);
+
+    GeomPtrs& d = args_.geom;
+    d.nf = nf;
+    d.nh = nh;
+    d.foff = static_cast<const int32_t*>(dev(sizeof(int32_t) * (nf + 1), g.foff.data()));
+    d.he_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * nh, g.he_face.data()));
+    d.twin = static_cast<const int32_t*>(dev(sizeof(int32_t) * nh, g.twin.data()));
+    d.sten = static_cast<const int32_t*>(dev(sizeof(int32_t) * sten.size(), sten.data()));
+    d.beta = static_cast<const double*>(dev(sizeof(double) * beta.size(), beta.data()));
+    d.hp = static_cast<const double*>(dev(sizeof(double) * nh, g.h_plus.data()));
+    d.hm = static_cast<const double*>(dev(sizeof(double) * nh, g.h_minus.data()));
+    d.bm = static_cast<const double*>(dev(sizeof(double) * nh, g.bm_dist.data()));
+    d.ec = static_cast<const double*>(dev(sizeof(double) * ec.size(), ec.data()));
+    d.en = static_cast<const double*>(dev(sizeof(double) * en.size(), en.data()));
+    d.arc = static_cast<const double*>(dev(sizeof(double) * nh, g.arc.data()));
+    d.area = static_cast<const double*>(dev(sizeof(double) * nf, g.area.data()));
+    d.fcn = static_cast<const double*>(dev(sizeof(double) * fcn.size(), fcn.data()));
+    d.acc = static_cast<const double*>(dev(sizeof(double) * acc.size(), acc.data()));
+    d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk.size(), blk.data()));
+    args_.n_flux_blocks = int(blk.size()) - 1;
+
+    PhysParams& P = args_.phys;
+    P.dim = dim_;
+    P.solver = solver_;
+    P.accretion_on = params_.accretion_on;
+    P.friction_on = params_.friction_on;
+    P.gam = gam_;
+    P.omega_ns = omega_ns_;
+    P.h0 = g.h0;
+    P.round_diff = g.round_diff;
+    P.e_acc = params_.e_acc;
+    P.alpha = params_.alpha;
+    const double fall_eff = 1;
+    P.dmdt_coef = dim_ == 5 ? -(fall_eff * params_.acc_rate * 4 * M_PI) / total_mass_ : 0.0;
+
+    args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
+    args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
+    const size_t up_elem = precision_ == kFP64 ? sizeof(double) : sizeof(float);
+    args_.UP = dev(up_elem * size_t(nh) * dim_, nullptr);
+
+    Scalars s{};
+    s.dt = 0;
+    s.t = 0;
+    s.dt0 = 0;
+    s.vmax_key = bits(1e-8);
+    s.dtx_key = host_dkey(1e20);
+    s.steps = 0;
+    s.stop = 0;
+    s.first_err = ~0ull;
+    args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars), &s));
+
+    n_diag_blocks_ = (nf + B - 1) / B;
+    d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
+}
+
+void Engine::enqueue_step(double dt0) {
+    fp64::enqueue_finalize_dt(args_, dt0, stream_);
+    if (precision_ == kFP64) fp64::enqueue_stages(args_, stream_);
+    else mixed::enqueue_stages(args_, stream_);
+}
+
+void Engine::step(double dt0) {
+    enqueue_step(dt0);
+    check(hipGetLastError(), "step launch");
+}
+
+void Engine::steps(double dt0, int n) {
+    if (n <= 0) return;
+    constexpr int kGraphSteps = 8;
+    if (n >= kGraphSteps) {
+        if (!graph_exec_ || graph_dt0_ != dt0 || graph_steps_ != kGraphSteps) {
+            if (graph_exec_) check(hipGraphExecDestroy(graph_exec_), "hipGraphExecDestroy");
+            if (graph_) check(hipGraphDestroy(graph_), "hipGraphDestroy");
+            graph_exec_ = nullptr;
+            graph_ = nullptr;
+            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            for (int k = 0; k < kGraphSteps; ++k) enqueue_step(dt0);
+            check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+            check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+            graph_dt0_ = dt0;
+            graph_steps_ = kGraphSteps;
+        }
+        while (n >= kGraphSteps) {
+            check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+            n -= kGraphSteps;
+        }
+    }
+    for (int k = 0; k < n; ++k) enqueue_step(dt0);
+    check(hipGetLastError(), "steps launch");
+}
+
+void Engine::synchronize() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+Scalars Engine::read_scalars() {
+    Scalars s;
+    check(hipMemcpyAsync(&s, args_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_), "read scalars");
+    synchronize();
+    return s;
+}
+
+double Engine::time() { return read_scalars().t; }
+double Engine::last_dt() { return read_scalars().dt; }
+uint64_t Engine::step_count() { return read_scalars().steps; }
+int Engine::stop_bits() { return read_scalars().stop; }
+uint64_t Engine::first_error() { return read_scalars().first_err; }
+
+void Engine::get_state(double* U_out) {
+    check(hipMemcpyAsync(U_out, args_.U, sizeof(double) * size_t(geo_.nf) * dim_, hipMemcpyDeviceToHost, stream_),
+          "read state");
+    synchronize();
+}
+
+void Engine::diagnostics(double out[5]) {
+    fp64::enqueue_diag(args_, d_partial_, stream_);
+    check(hipGetLastError(), "diag launch");
+    std::vector<double> part(5 * size_t(n_diag_blocks_));
+    check(hipMemcpyAsync(part.data(), d_partial_, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream_),
+          "read diagnostics");
+    synchronize();
+    for (int k = 0; k < 5; ++k) out[k] = 0;
+    for (int b = 0; b < n_diag_blocks_; ++b)
+        for (int k = 0; k < 5; ++k) out[k] += part[5 * size_t(b) + k];
+}
+
+}  // namespace splash

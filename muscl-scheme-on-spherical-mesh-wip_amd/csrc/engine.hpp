@@ -1,0 +1,90 @@
+// One SPLASH MUSCL run resident on one GPU: geometry tables, state and the
+// per-step launch sequence (optionally replayed from a captured HIP graph).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine_types.hpp"
+#include "geometry.hpp"
+#include "launch.hpp"
+#include "mesh.hpp"
+
+namespace splash {
+
+struct RunParams {
+    int32_t accretion_on = 0;
+    int32_t friction_on = 1;
+    double acc_rate = 3.25e-7;
+    double e_acc = 0.02;
+    double omega_acc_abs = 0.4;
+    double tilt_angle = 0.4;
+    double acc_width = 0.1;
+    double alpha = 1e-6;
+};
+
+class HipError : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+class Engine {
+public:
+    // U_in is [nf][dim] in mesh face order (slot 4 = energy for dim 5).
+    Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
+           const RunParams& params, int precision, int geometry_threads = 0);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void step(double dt0);              // enqueue one do_step
+    void steps(double dt0, int n);      // enqueue n do_steps from a captured graph
+    void synchronize();
+
+    double time();                      // synchronises
+    double last_dt();                   // synchronises
+    uint64_t step_count();
+    int stop_bits();
+    uint64_t first_error();
+    void get_state(double* U_out);      // [nf][dim], slot 4 = energy
+    void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
+
+    int32_t n_faces() const { return geo_.nf; }
+    int32_t n_halfedges() const { return geo_.nh; }
+    int dim() const { return dim_; }
+    int precision() const { return precision_; }
+    double area_coeff() const { return area_coeff_; }
+    double total_mass() const { return total_mass_; }
+    const Geometry& geometry() const { return geo_; }
+    hipStream_t stream() const { return stream_; }
+    const StepArgs& args() const { return args_; }
+
+private:
+    void enqueue_step(double dt0);
+    void upload();
+    Scalars read_scalars();
+
+    Geometry geo_;
+    int dim_, solver_, precision_;
+    double gam_, omega_ns_;
+    RunParams params_;
+    double area_coeff_ = 0, total_mass_ = 0;
+
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    StepArgs args_{};
+    double* d_partial_ = nullptr;
+    int n_diag_blocks_ = 0;
+
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    double graph_dt0_ = 0;
+    int graph_steps_ = 0;
+};
+
+}  // namespace splash

@@ -1,0 +1,62 @@
+// Plain-old-data types shared by the host engine and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace splash {
+
+enum SolverKind : int32_t { kHLLE = 0, kHLLE_P = 1, kHLLC = 2, kHLLCPLUS = 3 };
+enum Precision : int32_t { kFP64 = 0, kMixed = 1 };
+
+// Sticky error bits (the reference's stop_check causes).
+enum StopCode : int32_t {
+    kStopNaNFlux = 1,        // MUSCL_base.hpp:264-268
+    kStopNaNWaveSpeed = 2,   // adiabatic.hpp:671-676
+    kStopWaveBranch = 4,     // HLLC.hpp:114-119, HLLCplus.hpp:169-174, HLLE.hpp:51-55
+    kStopNegativeDt = 8,     // adiabatic.hpp:573-577 (g_eff < 0 somewhere)
+    kStopBadDim = 16,        // adiabatic.hpp:26-30, isothermal.hpp:18-22
+};
+
+// Device-resident per-run scalars.  The time step never leaves the GPU:
+// the next step's CFL/friction limits are reduced into vmax_key / dtx_key by
+// the previous step's last kernel and turned into dt by a one-thread kernel.
+struct Scalars {
+    double dt;                    // clamped dt of the current step
+    double t;                     // simulation time
+    double dt0;                   // requested dt
+    unsigned long long vmax_key;  // max(|v|, c_s) as the bits of a positive double
+    unsigned long long dtx_key;   // min friction dt limit, order-preserving key
+    unsigned long long steps;
+    int32_t stop;                 // StopCode bits
+    int32_t pad;
+    unsigned long long first_err; // (code << 40) | half-edge or face index, atomicMin
+};
+
+struct PhysParams {
+    int32_t dim, solver, accretion_on, friction_on;
+    double gam, omega_ns;
+    double h0, round_diff;
+    double e_acc, alpha;
+    double dmdt_coef;   // -(fall_eff*acc_rate*4*pi)/total_mass (adiabatic.hpp:463)
+};
+
+struct GeomPtrs {
+    int32_t nf, nh;
+    const int32_t* foff;      // [nf+1]
+    const int32_t* he_face;   // [nh]
+    const int32_t* twin;      // [nh]
+    const int32_t* sten;      // [nh][4]: plus0, plus1, minus0, minus1
+    const double* beta;       // [nh][4]: bp0, bp1, bm0, bm1
+    const double* hp;         // [nh]
+    const double* hm;         // [nh]
+    const double* bm;         // [nh]
+    const double* ec;         // [nh][4]: unit edge midpoint xyz, |ec|
+    const double* en;         // [nh][4]: edge normal xyz, chord length
+    const double* arc;        // [nh]
+    const double* area;       // [nf]
+    const double* fcn;        // [nf][4]: unit face centre xyz, unused
+    const double* acc;        // [nf][8]: accretion rate, omega_acc x r (3), r x (omega_acc x r) (3), unused
+    const int32_t* blk_face;  // [nblk+1] face ranges of the flux/update blocks
+};
+
+}  // namespace splash
