@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no GPU work: only host-side calls)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(splash_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from splash_amd import _lib
+    L = C.CDLL(_lib.LIB_PATH)
+    declared = _declared()
+    assert len(declared) >= 20
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    assert declared <= set(_lib.SIGNATURES), declared - set(_lib.SIGNATURES)
+
+
+def test_host_mesh_and_geometry_are_bit_identical_to_the_oracle():
+    import splash_amd as S
+    cases = [("hex", (4,), S.icosphere_hex), ("quad", (4,), S.quad_sphere), ("uv", (32, 16), S.uv_sphere),
+             ("ico", (3,), S.icosphere)]
+    for kind, args, gen in cases:
+        om, pm = O.Mesh(kind, *args), gen(*args)
+        assert np.array_equal(om.xyz.view(np.uint64), pm.vertices.view(np.uint64))
+        assert np.array_equal(om.foff, pm.face_offsets) and np.array_equal(om.fv, pm.face_vertices)
+        og, pg = O.Geometry(om), S.Geometry(pm, threads=4)
+        for f in ("fc", "normal", "area", "nb_edge", "twin", "ff_plus", "ff_minus", "beta_plus", "beta_minus",
+                  "h_plus", "h_minus", "bm_dist", "en"):
+            a, b = np.asarray(getattr(og, f)), np.asarray(getattr(pg, f))
+            assert a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8)), (kind, f)
+        assert og.h0 == pg.h0 and og.round_diff == pg.round_diff
+
+
+def test_argument_errors_are_codes_not_crashes():
+    import splash_amd as S
+    from splash_amd._lib import SplashError
+    mesh = S.icosphere_hex(1)
+    try:
+        S.MUSCL_HLLC(mesh, np.ones((mesh.n_faces, 5)), 4, 1.25)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("shape mismatch accepted")
+    # dim 4 on an adiabatic solver is rejected before any device work
+    try:
+        S.MUSCL_HLLC(mesh, np.ones((mesh.n_faces, 4)), 4, 1.25)
+    except SplashError as e:
+        assert e.code == -1
+    else:
+        raise AssertionError("dim 4 accepted by the adiabatic solver")
+    h = C.c_void_p()
+    rc = S._lib.lib().splash_mesh_icosphere_hex(-1, C.byref(h))
+    assert rc == -1
+    try:
+        S.icosphere_hex(99)
+    except SplashError as e:
+        assert e.code == -1
+    else:
+        raise AssertionError("level 99 accepted")

@@ -1,0 +1,276 @@
+"""Pins of the oracle's fluxes, limiter and step composition.
+
+Riemann fluxes are checked against textbook forms written in the local
+frame of an edge (Toro, "Riemann Solvers and Numerical Methods for Fluid
+Dynamics", 3rd ed.: PVRS wave speeds eqs. 10.67-10.69 with Davis/Einfeldt
+style bounds, HLL flux eq. 10.21, standard HLLC eqs. 10.37-10.41); the
+angular-momentum slots are r x (momentum flux).  The step is checked by
+invariants: conservation with sources off, exact rest-state preservation,
+thread-count independence, the CFL clamp and the stop_check triggers the
+survey measured on the reference build.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GAM = 2 - 1 / (4 / 3)
+
+
+def _frame(seed=0):
+    rng = np.random.default_rng(seed)
+    ec = rng.normal(size=3)
+    ec /= np.linalg.norm(ec)
+    t = rng.normal(size=3)
+    n = t - ec * (t @ ec)
+    n /= np.linalg.norm(n)
+    return ec, n, np.cross(ec, n)
+
+
+def _state(ec, sigma, v, pi, gam=GAM):
+    L = sigma * np.cross(ec, v)
+    E = pi / (gam - 1) + sigma * (v @ v) / 2
+    return np.array([sigma, *L, E])
+
+
+def _phys_flux(ec, n, sigma, v, pi, gam=GAM):
+    un = v @ n
+    E = pi / (gam - 1) + sigma * (v @ v) / 2
+    mom = sigma * v * un + pi * n
+    return np.array([sigma * un, *np.cross(ec, mom), (E + pi) * un])
+
+
+def _pvrs(sL, uL, pL, sR, uR, pR, gam=GAM):
+    aL, aR = math.sqrt(gam * pL / sL), math.sqrt(gam * pR / sR)
+    ps = max(0.0, 0.5 * (pL + pR) - 0.125 * (uR - uL) * (sL + sR) * (aL + aR))
+    q = lambda p: 1.0 if ps <= p else math.sqrt(1 + (gam + 1) / (2 * gam) * (ps / p - 1))
+    return uL - aL * q(pL), uR + aR * q(pR)
+
+
+def _flux(solver, uL, uR, ec, n):
+    F, err = O.flux_star(solver, GAM, uL, uR, ec, float(np.linalg.norm(ec)), n)
+    assert err == 0
+    return F
+
+
+def _close(a, b, tol=1e-12):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    assert np.max(np.abs(a - b)) <= tol * scale, (a, b)
+
+
+@pytest.mark.parametrize("solver", [O.HLLE_P, O.HLLC, O.HLLCPLUS])
+def test_consistency_equal_states(solver):
+    ec, n, t = _frame(1)
+    v = 0.2 * n + 0.3 * t
+    u = _state(ec, 1.3, v, 0.25)
+    _close(_flux(solver, u, u, ec, n), _phys_flux(ec, n, 1.3, v, 0.25))
+
+
+@pytest.mark.parametrize("solver", [O.HLLE_P, O.HLLC, O.HLLCPLUS])
+def test_supersonic_upwinding(solver):
+    ec, n, t = _frame(2)
+    vL, vR = 3.0 * n + 0.1 * t, 2.5 * n - 0.2 * t
+    uL, uR = _state(ec, 1.0, vL, 0.1), _state(ec, 0.8, vR, 0.12)
+    _close(_flux(solver, uL, uR, ec, n), _phys_flux(ec, n, 1.0, vL, 0.1))
+    # and flowing the other way
+    _close(_flux(solver, _state(ec, 1.0, -vL, 0.1), _state(ec, 0.8, -vR, 0.12), ec, n),
+           _phys_flux(ec, n, 0.8, -vR, 0.12))
+
+
+def test_hlle_p_matches_textbook():
+    ec, n, t = _frame(3)
+    sL, vL, pL = 1.2, 0.1 * n + 0.2 * t, 0.3
+    sR, vR, pR = 0.7, -0.15 * n - 0.1 * t, 0.12
+    SL, SR = _pvrs(sL, vL @ n, pL, sR, vR @ n, pR)
+    assert SL < 0 < SR
+    uL, uR = _state(ec, sL, vL, pL), _state(ec, sR, vR, pR)
+    FL, FR = _phys_flux(ec, n, sL, vL, pL), _phys_flux(ec, n, sR, vR, pR)
+    want = (SR * FL - SL * FR + SL * SR * (uR - uL)) / (SR - SL)
+    _close(_flux(O.HLLE_P, uL, uR, ec, n), want)
+
+
+def _toro_hllc(ec, n, sL, vL, pL, sR, vR, pR):
+    uL, uR = vL @ n, vR @ n
+    SL, SR = _pvrs(sL, uL, pL, sR, uR, pR)
+    Ss = (pR - pL + sL * uL * (SL - uL) - sR * uR * (SR - uR)) / (sL * (SL - uL) - sR * (SR - uR))
+
+    def star(s, v, p, S):
+        u = v @ n
+        E = p / (GAM - 1) + s * (v @ v) / 2
+        rho = s * (S - u) / (S - Ss)
+        m = rho * (Ss * n + (v - u * n))
+        e = rho * (E / s + (Ss - u) * (Ss + p / (s * (S - u))))
+        return np.array([rho, *np.cross(ec, m), e])
+
+    if SL >= 0:
+        return _phys_flux(ec, n, sL, vL, pL)
+    if Ss >= 0:
+        U = _state(ec, sL, vL, pL)
+        return _phys_flux(ec, n, sL, vL, pL) + SL * (star(sL, vL, pL, SL) - U)
+    if SR >= 0:
+        U = _state(ec, sR, vR, pR)
+        return _phys_flux(ec, n, sR, vR, pR) + SR * (star(sR, vR, pR, SR) - U)
+    return _phys_flux(ec, n, sR, vR, pR)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_hllc_matches_textbook(case):
+    ec, n, t = _frame(10 + case)
+    sL, vL, pL = [(1.2, 0.1 * n + 0.2 * t, 0.3), (0.9, -0.3 * n, 0.2), (1.0, 0.05 * n - 0.4 * t, 0.5)][case]
+    sR, vR, pR = [(0.7, -0.15 * n - 0.1 * t, 0.12), (1.1, -0.35 * n + 0.3 * t, 0.18),
+                  (1.4, 0.3 * n + 0.1 * t, 0.05)][case]
+    _close(_flux(O.HLLC, _state(ec, sL, vL, pL), _state(ec, sR, vR, pR), ec, n),
+           _toro_hllc(ec, n, sL, vL, pL, sR, vR, pR), tol=1e-11)
+
+
+def test_hllcplus_reduces_to_hllc_for_pure_shear():
+    """Equal pressure and normal velocity: g = 1 - 1**M = 0 and V_R - V_L = 0,
+    so the HLLC+ correction vanishes."""
+    ec, n, t = _frame(5)
+    uL = _state(ec, 1.0, 0.1 * n + 0.3 * t, 0.2)
+    uR = _state(ec, 0.6, 0.1 * n - 0.2 * t, 0.2)
+    _close(_flux(O.HLLCPLUS, uL, uR, ec, n), _flux(O.HLLC, uL, uR, ec, n), tol=1e-13)
+
+
+@pytest.mark.parametrize("solver", [O.HLLE_P, O.HLLC, O.HLLCPLUS])
+def test_flux_antisymmetry(solver):
+    """F(uL, uR, n) = -F(uR, uL, -n): the same physical edge seen from its two
+    faces (exact in exact arithmetic for all three fluxes)."""
+    ec, n, t = _frame(7)
+    uL = _state(ec, 1.2, 0.1 * n + 0.2 * t, 0.3)
+    uR = _state(ec, 0.7, -0.05 * n - 0.1 * t, 0.12)
+    F1 = _flux(solver, uL, uR, ec, n)
+    F2 = _flux(solver, uR, uL, ec, -n)
+    _close(F1, -F2, tol=1e-12)
+    if solver == O.HLLCPLUS:
+        assert np.max(np.abs(F1 - _flux(O.HLLC, uL, uR, ec, n))) > 1e-8
+
+
+def test_isothermal_hlle_textbook():
+    ec, n, t = _frame(9)
+    sL, vL, sR, vR = 1.1, 0.2 * n + 0.1 * t, 0.9, -0.1 * n
+    uL = np.array([sL, *(sL * np.cross(ec, vL)), 0.0])
+    uR = np.array([sR, *(sR * np.cross(ec, vR)), 0.0])
+    F, err = O.flux_star(O.HLLE, GAM, uL, uR, ec, float(np.linalg.norm(ec)), n)
+    assert err == 0
+    SL = min(vL @ n, vR @ n) - 1
+    SR = max(vL @ n, vR @ n) + 1
+
+    def f(s, v):
+        un = v @ n
+        return np.array([s * un, *np.cross(ec, s * v * un + s * n)])
+
+    want = (SR * f(sL, vL) - SL * f(sR, vR) + SL * SR * (uR[:4] - uL[:4])) / (SR - SL)
+    _close(F[:4], want)
+
+
+def _lim(r, nu=0.05, em=0.8, ep=1.5, N=6):
+    return O.limiter_adiabatic(np.full(5, r, dtype=float), nu, em, ep, N)[0]
+
+
+def test_limiter_closed_form_values():
+    assert _lim(0.0) == 0.0
+    assert _lim(-0.7) == 0.0
+    assert _lim(float("nan")) == 0.0
+    # r = 1: h(1) = tanh(0)^6 = 0, third-order branch = min(superbee, 1)
+    assert _lim(1.0, nu=0.01, em=0.8, ep=0.9) == 1.0          # superbee = max(min(1, 6.67), 0.9)
+    assert _lim(1.0, nu=0.2, em=0.8, ep=0.9) == pytest.approx(0.9, abs=1e-15)
+    # r = 2, nu = 0.05, N = 6: superbee = max(min(1, 0.8*2/0.6), min(2, 1.5)) = 1.5,
+    # third order = min(1.5, 1 + 1.05/3) = 1.35, h = tanh(10)^6
+    h = math.tanh(10.0) ** 6
+    assert _lim(2.0) == pytest.approx((1 - h) * 1.35 + h * 1.5, rel=1e-15)
+    # r = 0.5: superbee = max(min(1, 0.4/0.6), min(0.5, 1.5)) = 2/3,
+    # third order = min(2/3, 1 - 1.05/6) = 2/3, h = 1 - tanh(4*0.25*0.5**0.1)
+    h = 1 - math.tanh(0.25 * 4 * 0.5 ** 0.1)
+    assert _lim(0.5) == pytest.approx(2 / 3, rel=1e-15)
+    # r = 0.5, nu = 0.5: superbee = max(min(1, 0.4/6), 0.5) = 0.5; third = min(0.5, 1 - 1.5/6) = 0.5
+    assert _lim(0.5, nu=0.5) == pytest.approx(0.5, rel=1e-15)
+    # r = 0.3, nu = 0.5: superbee = max(min(1, 0.24/6), 0.3) = 0.3; third = min(0.3, 1 - 0.35) = 0.3
+    assert _lim(0.3, nu=0.5) == pytest.approx(0.3, rel=1e-15)
+    # r = 0.9, nu = 0.05: superbee = max(min(1, 0.72/0.6), 0.9) = 1; third = 1 - 0.035 = 0.965
+    h = 1 - math.tanh(4 * 0.81 * 0.1 ** 0.1)
+    assert _lim(0.9) == pytest.approx((1 - h) * (1 + 1.05 / 3 * -0.1) + h * 1.0, rel=1e-14)
+
+
+def _rotating(m, seed, omega, gam=GAM):
+    g = O.Geometry(m)
+    fc = g.fc / np.linalg.norm(g.fc, axis=1)[:, None]
+    rng = np.random.default_rng(seed)
+    sig = 1 + 0.1 * rng.standard_normal(m.nf)
+    v = np.cross(np.array([0, 0, omega]), fc) + 0.01 * rng.standard_normal((m.nf, 3))
+    v -= fc * np.sum(v * fc, axis=1)[:, None]
+    L = sig[:, None] * np.cross(fc, v)
+    E = 0.1 * sig / (gam - 1) + sig * np.sum(v * v, axis=1) / 2
+    return np.column_stack([sig, L, E]), g
+
+
+@pytest.mark.parametrize("solver", [O.HLLCPLUS, O.HLLC, O.HLLE_P])
+def test_conservation_without_sources(solver):
+    m = O.Mesh("hex", 3)
+    U, g = _rotating(m, 1, 0.3)
+    s = O.Solver(m, U, 5, solver, GAM, 0.0, threads=4)
+    d0 = s.diagnostics()
+    for _ in range(20):
+        s.do_step(1e-3)
+    d1 = s.diagnostics()
+    assert abs(d1[0] / d0[0] - 1) < 5e-14
+    assert abs(d1[4] / d0[4] - 1) < 5e-14
+    assert np.max(np.abs(d1[1:4] - d0[1:4])) < 5e-14 * np.max(np.abs(d0[1:4])) + 1e-16
+    assert np.max(np.abs(s.state() - U)) > 1e-6  # the flow did evolve
+
+
+@pytest.mark.parametrize("kind,args", [("hex", (3,)), ("quad", (3,)), ("uv", (24, 12))])
+def test_rest_state_is_preserved(kind, args):
+    m = O.Mesh(kind, *args)
+    U = np.tile([1.0, 0.0, 0.0, 0.0, 0.1 / (GAM - 1)], (m.nf, 1))
+    s = O.Solver(m, U, 5, O.HLLCPLUS, GAM, 0.0)
+    for _ in range(10):
+        s.do_step(1e-3)
+    assert np.max(np.abs(s.state() - U)) < 1e-13
+    assert not s.get_stop_check()
+
+
+def test_thread_count_independence():
+    m = O.Mesh("hex", 3)
+    U, _ = _rotating(m, 3, 0.2)
+    out = []
+    for thr in (1, 3, 8):
+        s = O.Solver(m, U, 5, O.HLLCPLUS, GAM, 0.0, threads=thr)
+        for _ in range(5):
+            s.do_step(1e-3)
+        out.append(s.state())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_cfl_clamp_uses_max_of_speed_and_sound_speed():
+    m = O.Mesh("hex", 2)
+    U, g = _rotating(m, 4, 0.2)
+    s = O.Solver(m, U, 5, O.HLLC, GAM, 0.0, O.make_params(alpha=0.0))
+    s.do_step(1.0)  # huge dt0: the CFL limit decides
+    fc = g.fc / np.linalg.norm(g.fc, axis=1)[:, None]
+    vel = np.cross(fc, U[:, 1:4]) / U[:, :1]
+    vn = np.linalg.norm(vel, axis=1)
+    p = (U[:, 4] - U[:, 0] * vn ** 2 / 2) * (GAM - 1)
+    vmax = max(1e-8, vn.max(), np.sqrt(GAM * p / U[:, 0]).max())
+    assert s.last_dt() == pytest.approx(0.1 * g.h0 / vmax, rel=1e-14)
+
+
+def test_stop_check_triggers():
+    # solid-body rotation with Omega = 0.5 makes g_eff = GM - |v|^2 < 0 at the
+    # equator: the friction dt limit goes negative on step 1 (the survey saw
+    # this on the reference build for icosphere_hex(5)); Omega = 0.3 does not.
+    m = O.Mesh("hex", 5)
+    for om, expect in ((0.5, True), (0.3, False)):
+        U, _ = _rotating(m, 5, om)
+        s = O.Solver(m, U, 5, O.HLLCPLUS, GAM, 0.0, threads=8)
+        s.do_step(1e-4)
+        assert s.get_stop_check() is expect
+    # a NaN in the input reaches the flux check
+    m = O.Mesh("hex", 2)
+    U, _ = _rotating(m, 6, 0.1)
+    U[3, 0] = float("nan")
+    s = O.Solver(m, U, 5, O.HLLC, GAM, 0.0)
+    s.do_step(1e-4)
+    assert s.get_stop_check()
