@@ -106,6 +106,11 @@ void splash_destroy(splash_ctx *ctx);
 int splash_do_step(splash_ctx *ctx, double dt0);
 /* n consecutive do_step(dt0) calls, replayed from a captured HIP graph. */
 int splash_do_steps(splash_ctx *ctx, double dt0, int n);
+/* Measurement extension: runs n eager do_step(dt0) calls with HIP events
+ * around every launch on the context's stream; ms_out[6] receives the mean
+ * duration in ms of [dt finalize, reconstruct (stage 1), flux+update
+ * (stage 1), reconstruct (stage 2), flux+update (stage 2), whole step]. */
+int splash_profile_step(splash_ctx *ctx, double dt0, int n, double *ms_out);
 /* Waits for every enqueued step. */
 int splash_synchronize(splash_ctx *ctx);
 

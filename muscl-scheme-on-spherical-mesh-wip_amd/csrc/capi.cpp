@@ -168,6 +168,15 @@ int splash_do_steps(splash_ctx* ctx, double dt0, int n) {
     });
 }
 
+int splash_profile_step(splash_ctx* ctx, double dt0, int n, double* ms_out) {
+    CTX_OR_FAIL;
+    if (n < 0 || !ms_out) return fail(SPLASH_EINVAL, "bad profile arguments");
+    return guarded([&] {
+        ctx->engine->profile(dt0, n, ms_out);
+        return SPLASH_OK;
+    });
+}
+
 int splash_synchronize(splash_ctx* ctx) {
     CTX_OR_FAIL;
     return guarded([&] {

@@ -15,18 +15,6 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-unsigned long long host_dkey(double d) {
-    unsigned long long b;
-    std::memcpy(&b, &d, 8);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-unsigned long long bits(double d) {
-    unsigned long long b;
-    std::memcpy(&b, &d, 8);
-    return b;
-}
-
 inline double norm3(double x, double y, double z) {
     double s = 0.0;
     s = s + x * x;
@@ -176,7 +164,8 @@ void Engine::upload() {
     d.fcn = static_cast<const double*>(dev(sizeof(double) * fcn.size(), fcn.data()));
     d.acc = static_cast<const double*>(dev(sizeof(double) * acc.size(), acc.data()));
     d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk.size(), blk.data()));
-    args_.n_flux_blocks = int(blk.size()) - 1;
+    d.n_blocks = int32_t(blk.size()) - 1;
+    d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
 
     PhysParams& P = args_.phys;
     P.dim = dim_;
@@ -201,8 +190,6 @@ void Engine::upload() {
     s.dt = 0;
     s.t = 0;
     s.dt0 = 0;
-    s.vmax_key = bits(1e-8);
-    s.dtx_key = host_dkey(1e20);
     s.steps = 0;
     s.stop = 0;
     s.first_err = ~0ull;
@@ -212,10 +199,55 @@ void Engine::upload() {
     d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
 }
 
+void Engine::enqueue_kernel(int which, double dt0) {
+    const bool f64 = precision_ == kFP64;
+    switch (which) {
+    case 0: fp64::enqueue_finalize_dt(args_, dt0, stream_); break;
+    case 1:
+    case 3:
+        if (f64) fp64::enqueue_reconstruct(args_, stream_);
+        else mixed::enqueue_reconstruct(args_, stream_);
+        break;
+    case 2:
+    case 4:
+        if (f64) fp64::enqueue_flux_update(args_, which / 2, stream_);
+        else mixed::enqueue_flux_update(args_, which / 2, stream_);
+        break;
+    }
+}
+
 void Engine::enqueue_step(double dt0) {
-    fp64::enqueue_finalize_dt(args_, dt0, stream_);
-    if (precision_ == kFP64) fp64::enqueue_stages(args_, stream_);
-    else mixed::enqueue_stages(args_, stream_);
+    for (int k = 0; k < kKernelsPerStep; ++k) enqueue_kernel(k, dt0);
+}
+
+void Engine::profile(double dt0, int n, double* ms) {
+    // HIP events around every launch of n eager steps on this run's stream
+    std::vector<hipEvent_t> ev(kKernelsPerStep + 1);
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    double acc[kKernelsPerStep + 1] = {0};
+    try {
+        for (int s = 0; s < n; ++s) {
+            check(hipEventRecord(ev[0], stream_), "hipEventRecord");
+            for (int k = 0; k < kKernelsPerStep; ++k) {
+                enqueue_kernel(k, dt0);
+                check(hipEventRecord(ev[k + 1], stream_), "hipEventRecord");
+            }
+            check(hipEventSynchronize(ev[kKernelsPerStep]), "hipEventSynchronize");
+            for (int k = 0; k < kKernelsPerStep; ++k) {
+                float t = 0;
+                check(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "hipEventElapsedTime");
+                acc[k] += t;
+            }
+            float t = 0;
+            check(hipEventElapsedTime(&t, ev[0], ev[kKernelsPerStep]), "hipEventElapsedTime");
+            acc[kKernelsPerStep] += t;
+        }
+    } catch (...) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k <= kKernelsPerStep; ++k) ms[k] = n > 0 ? acc[k] / n : 0.0;
 }
 
 void Engine::step(double dt0) {

@@ -45,6 +45,11 @@ public:
     void step(double dt0);              // enqueue one do_step
     void steps(double dt0, int n);      // enqueue n do_steps from a captured graph
     void synchronize();
+    // Runs n eager steps with HIP events around each launch; ms receives the
+    // average duration of [finalize_dt, reconstruct 1, flux_update 1,
+    // reconstruct 2, flux_update 2, whole step].
+    static constexpr int kKernelsPerStep = 5;
+    void profile(double dt0, int n, double* ms);
 
     double time();                      // synchronises
     double last_dt();                   // synchronises
@@ -66,6 +71,7 @@ public:
 
 private:
     void enqueue_step(double dt0);
+    void enqueue_kernel(int which, double dt0);
     void upload();
     Scalars read_scalars();
 

@@ -17,15 +17,13 @@ enum StopCode : int32_t {
     kStopBadDim = 16,        // adiabatic.hpp:26-30, isothermal.hpp:18-22
 };
 
-// Device-resident per-run scalars.  The time step never leaves the GPU:
-// the next step's CFL/friction limits are reduced into vmax_key / dtx_key by
-// the previous step's last kernel and turned into dt by a one-thread kernel.
+// Device-resident per-run scalars.  The time step never leaves the GPU: the
+// previous step's last kernel leaves per-block partials of the CFL/friction
+// limits (GeomPtrs::dt_partial) and a one-block kernel turns them into dt.
 struct Scalars {
     double dt;                    // clamped dt of the current step
     double t;                     // simulation time
     double dt0;                   // requested dt
-    unsigned long long vmax_key;  // max(|v|, c_s) as the bits of a positive double
-    unsigned long long dtx_key;   // min friction dt limit, order-preserving key
     unsigned long long steps;
     int32_t stop;                 // StopCode bits
     int32_t pad;
@@ -56,7 +54,9 @@ struct GeomPtrs {
     const double* area;       // [nf]
     const double* fcn;        // [nf][4]: unit face centre xyz, unused
     const double* acc;        // [nf][8]: accretion rate, omega_acc x r (3), r x (omega_acc x r) (3), unused
-    const int32_t* blk_face;  // [nblk+1] face ranges of the flux/update blocks
+    const int32_t* blk_face;  // [n_blocks+1] face ranges of the flux/update blocks
+    int32_t n_blocks;
+    double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
 };
 
 }  // namespace splash

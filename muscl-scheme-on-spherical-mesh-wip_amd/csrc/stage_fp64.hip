@@ -13,38 +13,42 @@ namespace fp64 {
 namespace {
 
 template <int DIM, int SOLVER>
-void stages(const StepArgs& a, hipStream_t s) {
-    const int gh = (a.geom.nh + kBlock - 1) / kBlock;
-    const double* W = a.W;
-    double* Wm = a.W;
+void flux_update(const StepArgs& a, int stage, hipStream_t s) {
     double* UP = static_cast<double*>(a.UP);
-    k_reconstruct<double, DIM><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, W, UP);
-    k_flux_update<double, DIM, SOLVER, 1><<<a.n_flux_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, Wm);
-    k_reconstruct<double, DIM><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, W, UP);
-    k_flux_update<double, DIM, SOLVER, 2><<<a.n_flux_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, Wm);
+    if (stage == 1)
+        k_flux_update<double, DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
+    else
+        k_flux_update<double, DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
 }
 
 }  // namespace
 
+void enqueue_reconstruct(const StepArgs& a, hipStream_t s) {
+    const int gh = (a.geom.nh + kBlock - 1) / kBlock;
+    double* UP = static_cast<double*>(a.UP);
+    if (a.phys.dim == 5) k_reconstruct<double, 5><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
+    else k_reconstruct<double, 4><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
+}
+
+void enqueue_flux_update(const StepArgs& a, int stage, hipStream_t s) {
+    switch (a.phys.solver) {
+    case kHLLE: flux_update<4, kHLLE>(a, stage, s); break;
+    case kHLLE_P: flux_update<5, kHLLE_P>(a, stage, s); break;
+    case kHLLC: flux_update<5, kHLLC>(a, stage, s); break;
+    default: flux_update<5, kHLLCPLUS>(a, stage, s); break;
+    }
+}
+
 void enqueue_prepare(const StepArgs& a, hipStream_t s) {
-    const int gf = (a.geom.nf + kBlock - 1) / kBlock;
-    if (a.phys.dim == 5) k_prepare<5><<<gf, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W);
-    else k_prepare<4><<<gf, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W);
+    if (a.phys.dim == 5) k_prepare<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W);
+    else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W);
 }
 
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s) {
-    if (a.phys.dim == 5) k_finalize_dt<5><<<1, 64, 0, s>>>(a.phys, a.sc, dt0);
-    else k_finalize_dt<4><<<1, 64, 0, s>>>(a.phys, a.sc, dt0);
+    if (a.phys.dim == 5) k_finalize_dt<5><<<1, kBlock, 0, s>>>(a.geom, a.phys, a.sc, dt0);
+    else k_finalize_dt<4><<<1, kBlock, 0, s>>>(a.geom, a.phys, a.sc, dt0);
 }
 
-void enqueue_stages(const StepArgs& a, hipStream_t s) {
-    switch (a.phys.solver) {
-    case kHLLE: stages<4, kHLLE>(a, s); break;
-    case kHLLE_P: stages<5, kHLLE_P>(a, s); break;
-    case kHLLC: stages<5, kHLLC>(a, s); break;
-    default: stages<5, kHLLCPLUS>(a, s); break;
-    }
-}
 
 void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s) {
     const int gf = (a.geom.nf + kBlock - 1) / kBlock;

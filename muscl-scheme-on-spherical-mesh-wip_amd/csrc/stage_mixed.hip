@@ -13,25 +13,32 @@ namespace mixed {
 namespace {
 
 template <int DIM, int SOLVER>
-void stages(const StepArgs& a, hipStream_t s) {
-    const int gh = (a.geom.nh + kBlock - 1) / kBlock;
+void flux_update(const StepArgs& a, int stage, hipStream_t s) {
     float* UP = static_cast<float*>(a.UP);
-    k_reconstruct<float, DIM><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
-    k_flux_update<float, DIM, SOLVER, 1><<<a.n_flux_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
-    k_reconstruct<float, DIM><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
-    k_flux_update<float, DIM, SOLVER, 2><<<a.n_flux_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
+    if (stage == 1)
+        k_flux_update<float, DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
+    else
+        k_flux_update<float, DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
 }
 
 }  // namespace
 
-void enqueue_stages(const StepArgs& a, hipStream_t s) {
+void enqueue_reconstruct(const StepArgs& a, hipStream_t s) {
+    const int gh = (a.geom.nh + kBlock - 1) / kBlock;
+    float* UP = static_cast<float*>(a.UP);
+    if (a.phys.dim == 5) k_reconstruct<float, 5><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
+    else k_reconstruct<float, 4><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
+}
+
+void enqueue_flux_update(const StepArgs& a, int stage, hipStream_t s) {
     switch (a.phys.solver) {
-    case kHLLE: stages<4, kHLLE>(a, s); break;
-    case kHLLE_P: stages<5, kHLLE_P>(a, s); break;
-    case kHLLC: stages<5, kHLLC>(a, s); break;
-    default: stages<5, kHLLCPLUS>(a, s); break;
+    case kHLLE: flux_update<4, kHLLE>(a, stage, s); break;
+    case kHLLE_P: flux_update<5, kHLLE_P>(a, stage, s); break;
+    case kHLLC: flux_update<5, kHLLC>(a, stage, s); break;
+    default: flux_update<5, kHLLCPLUS>(a, stage, s); break;
     }
 }
+
 
 }  // namespace mixed
 }  // namespace splash

@@ -167,6 +167,13 @@ class _MUSCL:
     def do_steps(self, dt0, n):
         check(lib().splash_do_steps(self._ctx, dt0, int(n)))
 
+    def profile_step(self, dt0, n):
+        """Mean ms per launch of [dt finalize, reconstruct 1, flux/update 1,
+        reconstruct 2, flux/update 2, whole step] over n eager steps."""
+        out = np.zeros(6)
+        check(lib().splash_profile_step(self._ctx, dt0, int(n), _dp(out)))
+        return out
+
     def synchronize(self):
         check(lib().splash_synchronize(self._ctx))
 

@@ -60,6 +60,7 @@ SIGNATURES = {
     "splash_destroy": (None, [C.c_void_p]),
     "splash_do_step": (C.c_int, [C.c_void_p, C.c_double]),
     "splash_do_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "splash_profile_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double)]),
     "splash_synchronize": (C.c_int, [C.c_void_p]),
     "splash_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "splash_last_dt": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
