@@ -26,19 +26,21 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 GAM = 2 - 1 / (4 / 3)
-KERNELS = ["finalize_dt", "reconstruct_1", "flux_update_1", "reconstruct_2", "flux_update_2"]
+KERNELS = ["finalize_dt", "stage_1", "stage_2"]
 
 
 def algorithmic_bytes(kernel, F, H, d=5):
     """Canonical algorithmic bytes of one launch: FP64 state and geometry,
-    every array read once and written once, no intermediates.  Per step the
-    four stage kernels sum to 2*(104*H + 36*F) + 40*d*F."""
-    if kernel.startswith("reconstruct"):
-        return 76 * H + 8 * d * F
-    if kernel == "flux_update_1":
-        return 28 * H + 36 * F + 8 * d * F
-    if kernel == "flux_update_2":
-        return 28 * H + 36 * F + 16 * d * F
+    every array read once and written once, no intermediates.  Per half-edge
+    104 B (stencil faces 16, betas 32, H+- 16, BM 8, neighbour 4, half of the
+    56 B per-edge normal/midpoint/arc), per face 36 B (area, unit centre,
+    offset) and one state pass is 8*d B per face: stage 1 reads U0 and writes
+    U_half, stage 2 reads U_half and U0 and writes U1.  The two stages sum to
+    2*(104*H + 36*F) + 40*d*F per step."""
+    if kernel == "stage_1":
+        return 104 * H + 36 * F + 2 * 8 * d * F
+    if kernel == "stage_2":
+        return 104 * H + 36 * F + 3 * 8 * d * F
     return 0
 
 
@@ -117,7 +119,7 @@ def main():
     value = F * args.steps / elmx
     value64 = F * args.steps / el64
 
-    k = int(np.argmax(prof[:5]))
+    k = int(np.argmax(prof[:3]))
     name = KERNELS[k]
     ab = algorithmic_bytes(name, F, H)
     achieved = ab / (prof[k] * 1e-3) / 1e9
@@ -145,10 +147,10 @@ def main():
         "linf_err_vs_fp64": linf,
         "fp64": {"value": value64, "ms_per_step": el64 / args.steps * 1e3,
                  "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof64))),
-                 "step_hbm_frac": step_bytes / (prof64[5] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                 "step_hbm_frac": step_bytes / (prof64[3] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
         "step_algorithmic_bytes": step_bytes,
-        "step_achieved_GBs": step_bytes / (prof[5] * 1e-3) / 1e9,
+        "step_achieved_GBs": step_bytes / (prof[3] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": ab},

@@ -107,9 +107,8 @@ int splash_do_step(splash_ctx *ctx, double dt0);
 /* n consecutive do_step(dt0) calls, replayed from a captured HIP graph. */
 int splash_do_steps(splash_ctx *ctx, double dt0, int n);
 /* Measurement extension: runs n eager do_step(dt0) calls with HIP events
- * around every launch on the context's stream; ms_out[6] receives the mean
- * duration in ms of [dt finalize, reconstruct (stage 1), flux+update
- * (stage 1), reconstruct (stage 2), flux+update (stage 2), whole step]. */
+ * around every launch on the context's stream; ms_out[4] receives the mean
+ * duration in ms of [dt finalize, stage 1, stage 2, whole step]. */
 int splash_profile_step(splash_ctx *ctx, double dt0, int n, double *ms_out);
 /* Waits for every enqueued step. */
 int splash_synchronize(splash_ctx *ctx);
@@ -157,6 +156,11 @@ typedef struct splash_geometry_view {
 int splash_geometry_create(const splash_mesh_desc *mesh, int threads, splash_geometry **out);
 int splash_geometry_get_view(const splash_geometry *g, splash_geometry_view *out);
 void splash_geometry_free(splash_geometry *g);
+
+/* Host-side check of the internal GPU layout of a mesh (Morton face order,
+ * 256-slot patches with their halos): builds and validates it and returns
+ * {blocks (multiple of 8), non-empty blocks, halo half-edges, max halo}. */
+int splash_layout_stats(const splash_mesh_desc *mesh, int32_t out[4]);
 
 /* Human-readable text of the last error on this host thread. */
 const char *splash_last_error_message(void);

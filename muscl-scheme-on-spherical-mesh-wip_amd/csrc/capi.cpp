@@ -1,6 +1,7 @@
 // extern "C" boundary (include/splash.h): converts exceptions to error codes.
 #include "../../include/splash.h"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -10,6 +11,7 @@
 
 #include "engine.hpp"
 #include "geometry.hpp"
+#include "layout.hpp"
 #include "mesh.hpp"
 
 struct splash_mesh {
@@ -289,6 +291,25 @@ int splash_geometry_get_view(const splash_geometry* g, splash_geometry_view* v) 
 }
 
 void splash_geometry_free(splash_geometry* g) { delete g; }
+
+int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[4]) {
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        const splash::Geometry g = splash::build_geometry(from_desc(mesh), 0);
+        const splash::Layout L = splash::build_layout(g, 256);
+        splash::validate_layout(g, L);
+        int32_t max_halo = 0, nonempty = 0;
+        for (int32_t b = 0; b < L.n_blocks(); ++b) {
+            max_halo = std::max(max_halo, L.halo_off[b + 1] - L.halo_off[b]);
+            nonempty += L.blk_face[b + 1] > L.blk_face[b];
+        }
+        out[0] = L.n_blocks();
+        out[1] = nonempty;
+        out[2] = int32_t(L.halo_he.size());
+        out[3] = max_halo;
+        return SPLASH_OK;
+    });
+}
 
 const char* splash_last_error_message(void) { return g_last_error.c_str(); }
 

@@ -1,7 +1,9 @@
 // Host side of one resident SPLASH run: uploads the stencil tables, keeps
 // the state on the GPU and enqueues the per-step kernel sequence.
 #include "engine.hpp"
+#include "layout.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -57,9 +59,10 @@ Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, do
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     try {
         upload();
-        std::vector<double> U(U_in, U_in + size_t(nf) * dim);
-        check(hipMemcpyAsync(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, stream_),
-              "upload state");
+        std::vector<double> U(size_t(nf) * dim);
+        for (int32_t k = 0; k < nf; ++k)
+            for (int c = 0; c < dim; ++c) U[size_t(k) * dim + c] = U_in[size_t(perm_[k]) * dim + c];
+        check(hipMemcpy(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice), "upload state");
         fp64::enqueue_prepare(args_, stream_);
         check(hipGetLastError(), "prepare launch");
         check(hipStreamSynchronize(stream_), "prepare");
@@ -81,35 +84,60 @@ Engine::~Engine() {
 void Engine::upload() {
     const Geometry& g = geo_;
     const int32_t nf = g.nf, nh = g.nh;
+    if (g.max_valence >= 32) throw std::runtime_error("face valence >= 32 is not supported");
+    {
+        Layout L = build_layout(g, fp64::block_size());
+        validate_layout(g, L);
+        perm_ = std::move(L.perm);
+        iperm_ = std::move(L.iperm);
+        foff_ = std::move(L.foff);
+        he_old_ = std::move(L.he_old);
+        he_new_ = std::move(L.he_new);
+        blk_face_ = std::move(L.blk_face);
+        halo_off_ = std::move(L.halo_off);
+        halo_he_ = std::move(L.halo_he);
+        twin_slot_ = std::move(L.twin_slot);
+    }
     auto dev = [&](size_t bytes, const void* host) -> void* {
         void* p = nullptr;
         check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
         allocations_.push_back(p);
-        if (host) check(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice), "upload");
+        if (host && bytes) check(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice), "upload");
         return p;
     };
 
-    std::vector<int32_t> sten(4 * size_t(nh));
-    std::vector<double> beta(4 * size_t(nh)), ec(4 * size_t(nh)), en(4 * size_t(nh));
-    for (int32_t h = 0; h < nh; ++h) {
-        sten[4 * size_t(h) + 0] = g.ff_plus[2 * size_t(h)];
-        sten[4 * size_t(h) + 1] = g.ff_plus[2 * size_t(h) + 1];
-        sten[4 * size_t(h) + 2] = g.ff_minus[2 * size_t(h)];
-        sten[4 * size_t(h) + 3] = g.ff_minus[2 * size_t(h) + 1];
-        beta[4 * size_t(h) + 0] = g.beta_plus[2 * size_t(h)];
-        beta[4 * size_t(h) + 1] = g.beta_plus[2 * size_t(h) + 1];
-        beta[4 * size_t(h) + 2] = g.beta_minus[2 * size_t(h)];
-        beta[4 * size_t(h) + 3] = g.beta_minus[2 * size_t(h) + 1];
+    std::vector<int32_t> he_fv(nh), sten(4 * size_t(nh));
+    std::vector<double> beta(4 * size_t(nh)), hsc(4 * size_t(nh)), ec(4 * size_t(nh)), en(4 * size_t(nh));
+    for (int32_t hn = 0; hn < nh; ++hn) {
+        const int32_t h = he_old_[hn];
+        const int32_t f = g.he_face[h];
+        he_fv[hn] = iperm_[f] | ((g.foff[f + 1] - g.foff[f]) << 27);
+        sten[4 * size_t(hn) + 0] = iperm_[g.ff_plus[2 * size_t(h)]];
+        sten[4 * size_t(hn) + 1] = iperm_[g.ff_plus[2 * size_t(h) + 1]];
+        sten[4 * size_t(hn) + 2] = iperm_[g.ff_minus[2 * size_t(h)]];
+        sten[4 * size_t(hn) + 3] = iperm_[g.ff_minus[2 * size_t(h) + 1]];
+        beta[4 * size_t(hn) + 0] = g.beta_plus[2 * size_t(h)];
+        beta[4 * size_t(hn) + 1] = g.beta_plus[2 * size_t(h) + 1];
+        beta[4 * size_t(hn) + 2] = g.beta_minus[2 * size_t(h)];
+        beta[4 * size_t(hn) + 3] = g.beta_minus[2 * size_t(h) + 1];
+        hsc[4 * size_t(hn) + 0] = g.h_plus[h];
+        hsc[4 * size_t(hn) + 1] = g.h_minus[h];
+        hsc[4 * size_t(hn) + 2] = g.bm_dist[h];
+        // adiabatic.hpp:757-758: distance(v_j, v_j1) / surface_area[n_face]
+        hsc[4 * size_t(hn) + 3] = g.arc[h] / g.area[f];
         for (int c = 0; c < 3; ++c) {
-            ec[4 * size_t(h) + c] = g.ec[3 * size_t(h) + c];
-            en[4 * size_t(h) + c] = g.en[3 * size_t(h) + c];
+            ec[4 * size_t(hn) + c] = g.ec[3 * size_t(h) + c];
+            en[4 * size_t(hn) + c] = g.en[3 * size_t(h) + c];
         }
-        ec[4 * size_t(h) + 3] = g.ecn[h];
-        en[4 * size_t(h) + 3] = g.chord[h];
+        ec[4 * size_t(hn) + 3] = g.ecn[h];
+        en[4 * size_t(hn) + 3] = g.chord[h];
     }
-    std::vector<double> fcn(4 * size_t(nf), 0.0);
-    for (int32_t f = 0; f < nf; ++f)
-        for (int c = 0; c < 3; ++c) fcn[4 * size_t(f) + c] = g.fcn[3 * size_t(f) + c];
+    std::vector<double> area(nf), fcn(4 * size_t(nf), 0.0);
+    for (int32_t k = 0; k < nf; ++k) {
+        const int32_t f = perm_[k];
+        area[k] = g.area[f];
+        for (int c = 0; c < 3; ++c) fcn[4 * size_t(k) + c] = g.fcn[3 * size_t(f) + c];
+    }
 
     // accretion profile per face (adiabatic.hpp:425-449), evaluated once with
     // the expressions the reference evaluates on every source() call
@@ -118,15 +146,15 @@ void Engine::upload() {
         const double mu = M_PI / 2;
         const double sigma = params_.acc_width / 2.355;
         const double tilt = params_.tilt_angle;
-        for (int32_t f = 0; f < nf; ++f) {
-            const double* r = &fcn[4 * size_t(f)];
+        for (int32_t k = 0; k < nf; ++k) {
+            const double* r = &fcn[4 * size_t(k)];
             const double theta_acc = std::acos(r[1] * std::sin(tilt) + r[2] * std::cos(tilt));
             const double rate = 1 / area_coeff_ * params_.acc_rate / std::sqrt(2 * M_PI * sigma * sigma) *
                                 std::exp(-(theta_acc - mu) * (theta_acc - mu) / (2 * sigma * sigma));
             const double ox = 0, oy = std::sin(tilt) * params_.omega_acc_abs, oz = std::cos(tilt) * params_.omega_acc_abs;
             // omxr = omega_acc x r ; rxv = r x omxr
             const double mx = oy * r[2] - oz * r[1], my = oz * r[0] - ox * r[2], mz = ox * r[1] - oy * r[0];
-            double* a = &acc[8 * size_t(f)];
+            double* a = &acc[8 * size_t(k)];
             a[0] = rate;
             a[1] = mx;
             a[2] = my;
@@ -137,34 +165,24 @@ void Engine::upload() {
         }
     }
 
-    // flux/update blocks: consecutive faces whose half-edges fit one block
-    const int B = fp64::block_size();
-    std::vector<int32_t> blk{0};
-    for (int32_t f = 0; f < nf; ++f) {
-        if (g.foff[f + 1] - g.foff[blk.back()] > B) blk.push_back(f);
-        if (g.foff[f + 1] - g.foff[f] > B) throw std::runtime_error("face valence exceeds the block size");
-    }
-    blk.push_back(nf);
-
     GeomPtrs& d = args_.geom;
     d.nf = nf;
     d.nh = nh;
-    d.foff = static_cast<const int32_t*>(dev(sizeof(int32_t) * (nf + 1), g.foff.data()));
-    d.he_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * nh, g.he_face.data()));
-    d.twin = static_cast<const int32_t*>(dev(sizeof(int32_t) * nh, g.twin.data()));
+    d.foff = static_cast<const int32_t*>(dev(sizeof(int32_t) * (nf + 1), foff_.data()));
+    d.he_fv = static_cast<const int32_t*>(dev(sizeof(int32_t) * nh, he_fv.data()));
     d.sten = static_cast<const int32_t*>(dev(sizeof(int32_t) * sten.size(), sten.data()));
     d.beta = static_cast<const double*>(dev(sizeof(double) * beta.size(), beta.data()));
-    d.hp = static_cast<const double*>(dev(sizeof(double) * nh, g.h_plus.data()));
-    d.hm = static_cast<const double*>(dev(sizeof(double) * nh, g.h_minus.data()));
-    d.bm = static_cast<const double*>(dev(sizeof(double) * nh, g.bm_dist.data()));
+    d.hsc = static_cast<const double*>(dev(sizeof(double) * hsc.size(), hsc.data()));
     d.ec = static_cast<const double*>(dev(sizeof(double) * ec.size(), ec.data()));
     d.en = static_cast<const double*>(dev(sizeof(double) * en.size(), en.data()));
-    d.arc = static_cast<const double*>(dev(sizeof(double) * nh, g.arc.data()));
-    d.area = static_cast<const double*>(dev(sizeof(double) * nf, g.area.data()));
+    d.twin_slot = static_cast<const int32_t*>(dev(sizeof(int32_t) * twin_slot_.size(), twin_slot_.data()));
+    d.area = static_cast<const double*>(dev(sizeof(double) * nf, area.data()));
     d.fcn = static_cast<const double*>(dev(sizeof(double) * fcn.size(), fcn.data()));
     d.acc = static_cast<const double*>(dev(sizeof(double) * acc.size(), acc.data()));
-    d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk.size(), blk.data()));
-    d.n_blocks = int32_t(blk.size()) - 1;
+    d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_face_.size(), blk_face_.data()));
+    d.halo_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_off_.size(), halo_off_.data()));
+    d.halo_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_he_.size(), halo_he_.data()));
+    d.n_blocks = int32_t(blk_face_.size()) - 1;
     d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
 
     PhysParams& P = args_.phys;
@@ -183,37 +201,21 @@ void Engine::upload() {
 
     args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
     args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
-    const size_t up_elem = precision_ == kFP64 ? sizeof(double) : sizeof(float);
-    args_.UP = dev(up_elem * size_t(nh) * dim_, nullptr);
+    args_.W2 = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
 
     Scalars s{};
-    s.dt = 0;
-    s.t = 0;
-    s.dt0 = 0;
-    s.steps = 0;
-    s.stop = 0;
     s.first_err = ~0ull;
     args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars), &s));
 
+    const int B = fp64::block_size();
     n_diag_blocks_ = (nf + B - 1) / B;
     d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
 }
 
 void Engine::enqueue_kernel(int which, double dt0) {
-    const bool f64 = precision_ == kFP64;
-    switch (which) {
-    case 0: fp64::enqueue_finalize_dt(args_, dt0, stream_); break;
-    case 1:
-    case 3:
-        if (f64) fp64::enqueue_reconstruct(args_, stream_);
-        else mixed::enqueue_reconstruct(args_, stream_);
-        break;
-    case 2:
-    case 4:
-        if (f64) fp64::enqueue_flux_update(args_, which / 2, stream_);
-        else mixed::enqueue_flux_update(args_, which / 2, stream_);
-        break;
-    }
+    if (which == 0) fp64::enqueue_finalize_dt(args_, dt0, stream_);
+    else if (precision_ == kFP64) fp64::enqueue_stage(args_, which, stream_);
+    else mixed::enqueue_stage(args_, which, stream_);
 }
 
 void Engine::enqueue_step(double dt0) {
@@ -293,12 +295,21 @@ double Engine::time() { return read_scalars().t; }
 double Engine::last_dt() { return read_scalars().dt; }
 uint64_t Engine::step_count() { return read_scalars().steps; }
 int Engine::stop_bits() { return read_scalars().stop; }
-uint64_t Engine::first_error() { return read_scalars().first_err; }
+uint64_t Engine::first_error() {
+    // kernels record internal indices: map back to the mesh's numbering
+    const uint64_t e = read_scalars().first_err;
+    if (e == ~0ull) return e;
+    const uint64_t code = e >> 40, where = e & 0xffffffffffull;
+    const uint64_t mapped = code == kStopNaNFlux ? uint64_t(perm_[where]) : uint64_t(he_old_[where]);
+    return (code << 40) | mapped;
+}
 
 void Engine::get_state(double* U_out) {
-    check(hipMemcpyAsync(U_out, args_.U, sizeof(double) * size_t(geo_.nf) * dim_, hipMemcpyDeviceToHost, stream_),
-          "read state");
+    std::vector<double> U(size_t(geo_.nf) * dim_);
+    check(hipMemcpyAsync(U.data(), args_.U, sizeof(double) * U.size(), hipMemcpyDeviceToHost, stream_), "read state");
     synchronize();
+    for (int32_t k = 0; k < geo_.nf; ++k)
+        for (int c = 0; c < dim_; ++c) U_out[size_t(perm_[k]) * dim_ + c] = U[size_t(k) * dim_ + c];
 }
 
 void Engine::diagnostics(double out[5]) {

@@ -46,9 +46,8 @@ public:
     void steps(double dt0, int n);      // enqueue n do_steps from a captured graph
     void synchronize();
     // Runs n eager steps with HIP events around each launch; ms receives the
-    // average duration of [finalize_dt, reconstruct 1, flux_update 1,
-    // reconstruct 2, flux_update 2, whole step].
-    static constexpr int kKernelsPerStep = 5;
+    // average duration of [finalize_dt, stage 1, stage 2, whole step].
+    static constexpr int kKernelsPerStep = 3;
     void profile(double dt0, int n, double* ms);
 
     double time();                      // synchronises
@@ -80,6 +79,12 @@ private:
     double gam_, omega_ns_;
     RunParams params_;
     double area_coeff_ = 0, total_mass_ = 0;
+
+    // internal layout: faces along a Morton curve, half-edges face-major
+    std::vector<int32_t> perm_, iperm_;      // internal -> mesh face, mesh -> internal
+    std::vector<int32_t> foff_;              // internal face offsets
+    std::vector<int32_t> he_old_, he_new_;   // internal -> mesh half-edge and back
+    std::vector<int32_t> blk_face_, halo_off_, halo_he_, twin_slot_;
 
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;

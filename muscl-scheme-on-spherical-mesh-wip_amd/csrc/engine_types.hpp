@@ -38,24 +38,27 @@ struct PhysParams {
     double dmdt_coef;   // -(fall_eff*acc_rate*4*pi)/total_mass (adiabatic.hpp:463)
 };
 
+// Device tables in the internal face order (faces sorted along a Morton
+// curve of their centres; half-edges face-major in that order).  Every
+// per-half-edge value is the one of the reference's [face][edge] table for
+// the same half-edge.
 struct GeomPtrs {
     int32_t nf, nh;
     const int32_t* foff;      // [nf+1]
-    const int32_t* he_face;   // [nh]
-    const int32_t* twin;      // [nh]
-    const int32_t* sten;      // [nh][4]: plus0, plus1, minus0, minus1
+    const int32_t* he_fv;     // [nh] owner face | valence << 27
+    const int32_t* sten;      // [nh][4]: plus0, plus1, minus0, minus1 (internal face ids)
     const double* beta;       // [nh][4]: bp0, bp1, bm0, bm1
-    const double* hp;         // [nh]
-    const double* hm;         // [nh]
-    const double* bm;         // [nh]
-    const double* ec;         // [nh][4]: unit edge midpoint xyz, |ec|
+    const double* hsc;        // [nh][4]: H+, H-, BM_dist, arc/area of the owner
+    const double* ec;         // [nh][4]: unit edge midpoint xyz, |ec| as recomputed
     const double* en;         // [nh][4]: edge normal xyz, chord length
-    const double* arc;        // [nh]
+    const int32_t* twin_slot; // [nh][2]: twin half-edge, its LDS slot in the owner's block
     const double* area;       // [nf]
     const double* fcn;        // [nf][4]: unit face centre xyz, unused
     const double* acc;        // [nf][8]: accretion rate, omega_acc x r (3), r x (omega_acc x r) (3), unused
-    const int32_t* blk_face;  // [n_blocks+1] face ranges of the flux/update blocks
-    int32_t n_blocks;
+    const int32_t* blk_face;  // [n_blocks+1] face range of each stage block
+    const int32_t* halo_off;  // [n_blocks+1] range of each block's halo in halo_he
+    const int32_t* halo_he;   // outside twins of each block's boundary half-edges
+    int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
 };
 

@@ -13,22 +13,20 @@ struct StepArgs {
     PhysParams phys;
     Scalars* sc;
     double* U;  // [nf][dim] E-form state
-    double* W;  // [nf][dim] p-form stage input
-    void* UP;   // [nh][dim] reconstructed edge states (double or float)
+    double* W;   // [nf][dim] p-form input of stage 1 (and output of stage 2)
+    double* W2;  // [nf][dim] p-form input of stage 2
 };
 
 namespace fp64 {
 void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s);
-void enqueue_reconstruct(const StepArgs& a, hipStream_t s);
-void enqueue_flux_update(const StepArgs& a, int stage, hipStream_t s);
+void enqueue_stage(const StepArgs& a, int stage, hipStream_t s);
 void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s);
 int block_size();
 }  // namespace fp64
 
 namespace mixed {
-void enqueue_reconstruct(const StepArgs& a, hipStream_t s);
-void enqueue_flux_update(const StepArgs& a, int stage, hipStream_t s);
+void enqueue_stage(const StepArgs& a, int stage, hipStream_t s);
 }  // namespace mixed
 
 }  // namespace splash

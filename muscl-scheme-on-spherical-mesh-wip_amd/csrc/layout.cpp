@@ -1,0 +1,141 @@
+#include "layout.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace splash {
+
+namespace {
+
+// spread the low 21 bits of x to every third bit (3-D Morton interleave)
+uint64_t spread3(uint64_t x) {
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+void require(bool ok, const char* what) {
+    if (!ok) throw std::logic_error(std::string("layout check failed: ") + what);
+}
+
+}  // namespace
+
+Layout build_layout(const Geometry& g, int32_t slots) {
+    const int32_t nf = g.nf, nh = g.nh;
+    Layout L;
+    L.slots = slots;
+    std::vector<uint64_t> code(nf);
+    for (int32_t f = 0; f < nf; ++f) {
+        uint64_t c = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double v = (g.fcn[3 * size_t(f) + k] + 1.0) * 0.5 * double((1u << 21) - 1);
+            const uint64_t q = uint64_t(std::min(std::max(v, 0.0), double((1u << 21) - 1)));
+            c |= spread3(q) << k;
+        }
+        code[f] = c;
+    }
+    L.perm.resize(nf);
+    for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
+    std::stable_sort(L.perm.begin(), L.perm.end(), [&](int32_t x, int32_t y) { return code[x] < code[y]; });
+    L.iperm.resize(nf);
+    for (int32_t k = 0; k < nf; ++k) L.iperm[L.perm[k]] = k;
+
+    L.foff.assign(nf + 1, 0);
+    for (int32_t k = 0; k < nf; ++k) L.foff[k + 1] = L.foff[k] + (g.foff[L.perm[k] + 1] - g.foff[L.perm[k]]);
+    L.he_old.resize(nh);
+    L.he_new.resize(nh);
+    for (int32_t k = 0; k < nf; ++k) {
+        const int32_t f = L.perm[k];
+        for (int32_t j = 0; j < g.foff[f + 1] - g.foff[f]; ++j) {
+            L.he_old[L.foff[k] + j] = g.foff[f] + j;
+            L.he_new[g.foff[f] + j] = L.foff[k] + j;
+        }
+    }
+    auto twin_owner = [&](int32_t hn) { return L.iperm[g.he_face[g.twin[L.he_old[hn]]]]; };
+
+    // greedy patches: add faces while own half-edges + halo fit the slots
+    L.blk_face.assign(1, 0);
+    int32_t own = 0, halo = 0;
+    for (int32_t k = 0; k < nf; ++k) {
+        const int32_t val = L.foff[k + 1] - L.foff[k];
+        if (val > slots) throw std::runtime_error("face valence exceeds the block's LDS slots");
+        int32_t dh = 0;
+        const int32_t f0 = L.blk_face.back();
+        for (int32_t hn = L.foff[k]; hn < L.foff[k + 1]; ++hn) {
+            const int32_t o = twin_owner(hn);
+            dh += (o >= f0 && o < k) ? -1 : 1;
+        }
+        if (own + val + halo + dh > slots) {
+            L.blk_face.push_back(k);
+            own = 0;
+            halo = 0;
+            dh = val;
+        }
+        own += val;
+        halo += dh;
+    }
+    L.blk_face.push_back(nf);
+    while ((L.blk_face.size() - 1) % 8) L.blk_face.push_back(nf);
+
+    const int32_t nb = L.n_blocks();
+    L.twin_slot.assign(2 * size_t(nh), 0);
+    L.halo_off.assign(1, 0);
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+        const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
+        int32_t n_halo = 0;
+        for (int32_t hn = h0; hn < h1; ++hn) {
+            const int32_t tw = L.he_new[g.twin[L.he_old[hn]]];
+            const int32_t o = twin_owner(hn);
+            L.twin_slot[2 * size_t(hn)] = tw;
+            if (o >= f0 && o < f1) {
+                L.twin_slot[2 * size_t(hn) + 1] = tw - h0;
+            } else {
+                L.twin_slot[2 * size_t(hn) + 1] = (h1 - h0) + n_halo++;
+                L.halo_he.push_back(tw);
+            }
+        }
+        L.halo_off.push_back(int32_t(L.halo_he.size()));
+    }
+    return L;
+}
+
+void validate_layout(const Geometry& g, const Layout& L) {
+    const int32_t nf = g.nf, nh = g.nh, nb = L.n_blocks();
+    require(int32_t(L.perm.size()) == nf && int32_t(L.foff.size()) == nf + 1, "sizes");
+    std::vector<char> seen(nf, 0);
+    for (int32_t k = 0; k < nf; ++k) {
+        require(L.perm[k] >= 0 && L.perm[k] < nf && !seen[L.perm[k]], "perm is a permutation");
+        seen[L.perm[k]] = 1;
+        require(L.iperm[L.perm[k]] == k, "iperm inverts perm");
+    }
+    require(L.foff[0] == 0 && L.foff[nf] == nh, "foff covers the half-edges");
+    for (int32_t hn = 0; hn < nh; ++hn) require(L.he_new[L.he_old[hn]] == hn, "half-edge maps are inverse");
+    require(nb % 8 == 0 && L.blk_face[0] == 0 && L.blk_face[nb] == nf, "blocks cover the faces");
+    require(int32_t(L.halo_off.size()) == nb + 1 && L.halo_off[nb] == int32_t(L.halo_he.size()), "halo offsets");
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+        require(f0 <= f1 && f1 - f0 <= L.slots, "faces per block <= threads");
+        const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
+        const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
+        require(n_halo >= 0 && n_own + n_halo <= L.slots, "own + halo half-edges fit the LDS slots");
+        for (int32_t q = L.halo_off[b]; q < L.halo_off[b + 1]; ++q) {
+            const int32_t he = L.halo_he[q];
+            require(he >= 0 && he < nh && (he < h0 || he >= h1), "halo half-edges are outside the block");
+        }
+        for (int32_t hn = h0; hn < h1; ++hn) {
+            const int32_t tw = L.twin_slot[2 * size_t(hn)], slot = L.twin_slot[2 * size_t(hn) + 1];
+            require(tw >= 0 && tw < nh && L.twin_slot[2 * size_t(tw)] == hn, "twin is an involution");
+            require(slot >= 0 && slot < n_own + n_halo, "twin slot in range");
+            const int32_t at = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+            require(at == tw, "twin slot holds the twin");
+        }
+    }
+}
+
+}  // namespace splash

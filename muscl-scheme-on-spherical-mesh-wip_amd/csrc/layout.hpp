@@ -1,0 +1,34 @@
+// Internal GPU layout of a mesh: faces reordered along a Morton curve of
+// their centres, half-edges face-major in that order, and the partition of
+// faces into patches ("blocks") that one workgroup advances through an RK
+// stage with its half-edges plus the outside twins of its boundary
+// half-edges (the halo) resident in LDS.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace splash {
+
+struct Layout {
+    int32_t slots = 0;                 // LDS half-edge slots per block
+    std::vector<int32_t> perm, iperm;  // internal -> mesh face, mesh -> internal
+    std::vector<int32_t> foff;         // [nf+1] internal face offsets
+    std::vector<int32_t> he_old;       // internal -> mesh half-edge
+    std::vector<int32_t> he_new;       // mesh -> internal half-edge
+    std::vector<int32_t> blk_face;     // [n_blocks+1], n_blocks a multiple of 8
+    std::vector<int32_t> halo_off;     // [n_blocks+1]
+    std::vector<int32_t> halo_he;      // internal half-edge ids
+    std::vector<int32_t> twin_slot;    // [nh][2]: internal twin, its slot in the owner's block
+    int32_t n_blocks() const { return int32_t(blk_face.size()) - 1; }
+};
+
+Layout build_layout(const Geometry& g, int32_t slots);
+
+// Checks every index the stage kernel dereferences (throws std::logic_error
+// on the first violation).
+void validate_layout(const Geometry& g, const Layout& L);
+
+}  // namespace splash

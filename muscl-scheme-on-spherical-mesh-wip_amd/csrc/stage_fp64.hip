@@ -13,29 +13,21 @@ namespace fp64 {
 namespace {
 
 template <int DIM, int SOLVER>
-void flux_update(const StepArgs& a, int stage, hipStream_t s) {
-    double* UP = static_cast<double*>(a.UP);
+void stage(const StepArgs& a, int stage, hipStream_t s) {
     if (stage == 1)
-        k_flux_update<double, DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
+        k_stage<double, DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W, a.W2);
     else
-        k_flux_update<double, DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, UP, a.U, a.W);
+        k_stage<double, DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W2, a.W);
 }
 
 }  // namespace
 
-void enqueue_reconstruct(const StepArgs& a, hipStream_t s) {
-    const int gh = (a.geom.nh + kBlock - 1) / kBlock;
-    double* UP = static_cast<double*>(a.UP);
-    if (a.phys.dim == 5) k_reconstruct<double, 5><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
-    else k_reconstruct<double, 4><<<gh, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.W, UP);
-}
-
-void enqueue_flux_update(const StepArgs& a, int stage, hipStream_t s) {
+void enqueue_stage(const StepArgs& a, int st, hipStream_t s) {
     switch (a.phys.solver) {
-    case kHLLE: flux_update<4, kHLLE>(a, stage, s); break;
-    case kHLLE_P: flux_update<5, kHLLE_P>(a, stage, s); break;
-    case kHLLC: flux_update<5, kHLLC>(a, stage, s); break;
-    default: flux_update<5, kHLLCPLUS>(a, stage, s); break;
+    case kHLLE: stage<4, kHLLE>(a, st, s); break;
+    case kHLLE_P: stage<5, kHLLE_P>(a, st, s); break;
+    case kHLLC: stage<5, kHLLC>(a, st, s); break;
+    default: stage<5, kHLLCPLUS>(a, st, s); break;
     }
 }
 

@@ -125,6 +125,15 @@ class Geometry:
             lib().splash_geometry_free(h)
 
 
+def layout_stats(mesh):
+    """(blocks, non-empty blocks, halo half-edges, max halo) of the validated
+    internal GPU layout of `mesh` (host only)."""
+    out = (C.c_int32 * 4)()
+    d = mesh.desc()
+    check(lib().splash_layout_stats(C.byref(d), out))
+    return tuple(out)
+
+
 def default_params(**overrides):
     """The reference's shipped input/parameters.json physics keys, accretion off."""
     p = Params()
@@ -168,9 +177,9 @@ class _MUSCL:
         check(lib().splash_do_steps(self._ctx, dt0, int(n)))
 
     def profile_step(self, dt0, n):
-        """Mean ms per launch of [dt finalize, reconstruct 1, flux/update 1,
-        reconstruct 2, flux/update 2, whole step] over n eager steps."""
-        out = np.zeros(6)
+        """Mean ms per launch of [dt finalize, stage 1, stage 2, whole step]
+        over n eager steps (HIP events on the run's stream)."""
+        out = np.zeros(4)
         check(lib().splash_profile_step(self._ctx, dt0, int(n), _dp(out)))
         return out
 

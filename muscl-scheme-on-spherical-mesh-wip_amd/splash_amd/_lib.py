@@ -73,6 +73,7 @@ SIGNATURES = {
     "splash_geometry_create": (C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_void_p)]),
     "splash_geometry_get_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
     "splash_geometry_free": (None, [C.c_void_p]),
+    "splash_layout_stats": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_int32)]),
     "splash_last_error_message": (C.c_char_p, []),
 }
 

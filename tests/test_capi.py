@@ -73,3 +73,14 @@ def test_argument_errors_are_codes_not_crashes():
         assert e.code == -1
     else:
         raise AssertionError("level 99 accepted")
+
+
+def test_gpu_layout_validates_on_every_mesh_family():
+    """The stage kernel's patch/halo layout passes its host-side index checks
+    (blocks a multiple of 8, own + halo half-edges within the LDS slots,
+    twin slots holding the twins) on every generator."""
+    import splash_amd as S
+    for mesh in (S.icosphere_hex(5), S.quad_sphere(5), S.uv_sphere(96, 48), S.icosphere(4), S.icosphere_hex(0)):
+        blocks, nonempty, halo, max_halo = S.layout_stats(mesh)
+        assert blocks % 8 == 0 and 0 < nonempty <= blocks
+        assert 0 < halo < mesh.n_halfedges and max_halo < 256
