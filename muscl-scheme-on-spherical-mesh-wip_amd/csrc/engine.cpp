@@ -182,6 +182,32 @@ void Engine::upload() {
     d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_face_.size(), blk_face_.data()));
     d.halo_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_off_.size(), halo_off_.data()));
     d.halo_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_he_.size(), halo_he_.data()));
+    if (precision_ == kMixed) {
+        // FP32 records of the mixed kernel: beta, {1/H+, 1/H-, BM, arc/area},
+        // {ec, |ec|}, {en, chord}
+        std::vector<float> b32(4 * size_t(nh)), h32(4 * size_t(nh)), e32(4 * size_t(nh)), n32(4 * size_t(nh));
+        for (size_t q = 0; q < 4 * size_t(nh); ++q) {
+            b32[q] = float(beta[q]);
+            e32[q] = float(ec[q]);
+            n32[q] = float(en[q]);
+        }
+        for (int32_t hn = 0; hn < nh; ++hn) {
+            h32[4 * size_t(hn) + 0] = float(1.0 / hsc[4 * size_t(hn) + 0]);
+            h32[4 * size_t(hn) + 1] = float(1.0 / hsc[4 * size_t(hn) + 1]);
+            h32[4 * size_t(hn) + 2] = float(hsc[4 * size_t(hn) + 2]);
+            h32[4 * size_t(hn) + 3] = float(hsc[4 * size_t(hn) + 3]);
+        }
+        d.beta32 = static_cast<const float*>(dev(sizeof(float) * b32.size(), b32.data()));
+        d.hsc32 = static_cast<const float*>(dev(sizeof(float) * h32.size(), h32.data()));
+        d.ec32 = static_cast<const float*>(dev(sizeof(float) * e32.size(), e32.data()));
+        d.en32 = static_cast<const float*>(dev(sizeof(float) * n32.size(), n32.data()));
+        const size_t row = dim_ == 5 ? 8 : 4;
+        args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
+        args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
+    } else {
+        d.beta32 = d.hsc32 = d.ec32 = d.en32 = nullptr;
+        args_.W32 = args_.W32b = nullptr;
+    }
     d.n_blocks = int32_t(blk_face_.size()) - 1;
     d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
 

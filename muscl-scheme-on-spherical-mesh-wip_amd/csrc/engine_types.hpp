@@ -58,6 +58,12 @@ struct GeomPtrs {
     const int32_t* blk_face;  // [n_blocks+1] face range of each stage block
     const int32_t* halo_off;  // [n_blocks+1] range of each block's halo in halo_he
     const int32_t* halo_he;   // outside twins of each block's boundary half-edges
+    // FP32 copies for the mixed-precision kernel (null in FP64 runs); each
+    // holds exactly the value the FP64 table converts to at load time
+    const float* beta32;      // [nh][4]
+    const float* hsc32;       // [nh][4]: H+, H-, BM_dist, arc/area
+    const float* ec32;        // [nh][4]: unit edge midpoint xyz, |ec|
+    const float* en32;        // [nh][4]: edge normal xyz, chord length
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
 };

@@ -32,8 +32,8 @@ void enqueue_stage(const StepArgs& a, int st, hipStream_t s) {
 }
 
 void enqueue_prepare(const StepArgs& a, hipStream_t s) {
-    if (a.phys.dim == 5) k_prepare<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W);
-    else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W);
+    if (a.phys.dim == 5) k_prepare<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W, a.W32);
+    else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W, a.W32);
 }
 
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s) {
