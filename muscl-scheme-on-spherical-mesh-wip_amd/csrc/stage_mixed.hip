@@ -8,6 +8,20 @@
 
 namespace splash {
 namespace mixed {
+
+// FP32 math of the mixed path on the native v_sqrt / v_log / v_exp / v_rcp
+// instructions (about 1 ulp each, far inside the path's 1e-5 error budget)
+__device__ __forceinline__ float m_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float m_pow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));  // x > 0 at every call site
+}
+__device__ __forceinline__ float m_tanh(float x) {
+    // tanh|x| = 1 - 2 / (e^{2|x|} + 1), sign restored; saturates to 1 for large |x|
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * fabsf(x));  // 2/ln 2
+    return copysignf(1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f), x);
+}
+
 #include "stage_kernels.inc"
 
 // ----------------------------------------------------------------------
@@ -63,27 +77,27 @@ __device__ __forceinline__ void reconstruct32(const GeomPtrs& g, const PhysParam
     for (int k = 0; k < DIM; ++k) {
         pp[k] = (bt.x * a[k] + bt.y * b[k] - wi[k]) * hs.x;
         const float pm = (wi[k] - (bt.z * c[k] + bt.w * d[k])) * hs.y;
-        r[k] = pm / pp[k];
+        r[k] = m_div(pm, pp[k]);
     }
     const V3<float> ec = v3(e4.x, e4.y, e4.z);
     const V3<float> en = v3(n4.x, n4.y, n4.z);
-    const float eta_m = hs.z * hs.y, eta_p = hs.z * hs.x;  // (H-/BM, H+/BM) as BM / H
+    const float eta_m = __builtin_amdgcn_rcpf(hs.z * hs.y), eta_p = __builtin_amdgcn_rcpf(hs.z * hs.x);  // H-+/BM
     if constexpr (DIM == 5) {
         const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0] * e4.w);
-        const float cs = sqrtf(float(P.gam) * wi[4] / wi[0]);
+        const float cs = m_sqrt(m_div(float(P.gam) * wi[4], wi[0]));
         const float nu = (cs + dot(vel, en)) * dt * hs.w;
-        limiter_adiabatic<float, DIM>(r, nu, 1.0f / eta_m, 1.0f / eta_p, valence, lim);
+        limiter_adiabatic<float, DIM>(r, nu, eta_m, eta_p, valence, lim);
     } else {
         const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0]);
         const float nu = (1.0f + dot(vel, en)) * dt * hs.w;
-        limiter_isothermal<float, DIM>(r, nu, 1.0f / eta_m, valence, lim);
+        limiter_isothermal<float, DIM>(r, nu, eta_m, valence, lim);
     }
 #pragma unroll
     for (int k = 0; k < DIM; ++k) out[k] = wi[k] + pp[k] * lim[k] * hs.z;
     if constexpr (DIM == 5) {
         // E_fc (MUSCL_base.hpp:541-562)
         const V3<float> vel = cross(ec, v3(out[1], out[2], out[3])) / (-out[0]);
-        out[4] = out[4] / float(P.gam - 1) + out[0] * dot(vel, vel) * 0.5f;
+        out[4] = out[4] * float(1 / (P.gam - 1)) + out[0] * dot(vel, vel) * 0.5f;
     }
 }
 
