@@ -1,0 +1,415 @@
+// Drop-in C++ classes for the SPLASH MUSCL solvers, running on the GPU
+// through the C ABI of include/splash.h.
+//
+// Names, constructor arguments and methods follow the reference:
+//   MUSCL_HLLE(mesh, U_in, dim, gam, threads)                       HLLE.hpp:13
+//   MUSCL_HLLE_p / MUSCL_HLLC / MUSCL_HLLCplus(mesh, U_in, dim, gam,
+//       omega_ns, accretion_on, threads)                           HLLE_p.hpp:11, HLLC.hpp:11, HLLCplus.hpp:11
+//   README form MUSCL_HLLCplus(mesh, U_in, dim, gam, omega_ns)      README.md:23
+//   do_step(dt) / time() / get_stop_check()                         MUSCL_base.hpp:97, :227, :232
+//   write_face_centers / write_faces / write_vertices               MUSCL_geometry.hpp:1069-1117
+//   write_t_rho / write_t_p / write_t_curl / write_t_omega_z /
+//   write_t_L / get_light_curves / write_final_state                adiabatic.hpp:101-348
+// and keep the reference's side effects: the adiabatic constructor reads
+// input/parameters.json (throwing when it is missing or malformed) and both
+// physics constructors truncate the results/*.dat series; do_step prints the
+// per-step diagnostics line (set_verbose(false) silences it).
+//
+// `threads` (the OpenMP team size of the reference) is accepted and ignored.
+// Extensions: get_state(), do_steps(dt, n), set_verbose(), precision modes.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../splash.h"
+
+namespace splash {
+
+inline void check_rc(int rc) {
+    if (rc != SPLASH_OK) throw std::runtime_error(std::string("splash: ") + splash_last_error_message());
+}
+
+// The flattened pmp::SurfaceMesh the solvers consume (MUSCL_geometry.hpp:66-88).
+class SurfaceMesh {
+public:
+    SurfaceMesh() = default;
+    SurfaceMesh(std::vector<double> xyz, std::vector<int32_t> foff, std::vector<int32_t> fv)
+        : xyz_(std::move(xyz)), foff_(std::move(foff)), fv_(std::move(fv)) {}
+    size_t n_vertices() const { return xyz_.size() / 3; }
+    size_t n_faces() const { return foff_.empty() ? 0 : foff_.size() - 1; }
+    const std::vector<double>& vertices() const { return xyz_; }
+    const std::vector<int32_t>& face_offsets() const { return foff_; }
+    const std::vector<int32_t>& face_vertices() const { return fv_; }
+    splash_mesh_desc desc() const {
+        return splash_mesh_desc{int32_t(n_vertices()), int32_t(n_faces()), xyz_.data(), foff_.data(), fv_.data()};
+    }
+
+    static SurfaceMesh from_generator(int rc, splash_mesh* m) {
+        check_rc(rc);
+        splash_mesh_desc d;
+        splash_mesh_view(m, &d);
+        SurfaceMesh out(std::vector<double>(d.vertices, d.vertices + 3 * size_t(d.n_vertices)),
+                        std::vector<int32_t>(d.face_offsets, d.face_offsets + d.n_faces + 1),
+                        std::vector<int32_t>(d.face_vertices, d.face_vertices + d.face_offsets[d.n_faces]));
+        splash_mesh_free(m);
+        return out;
+    }
+
+private:
+    std::vector<double> xyz_;
+    std::vector<int32_t> foff_, fv_;
+};
+
+// sph_gen.h:215-295
+inline SurfaceMesh uv_sphere(int n_slices, int n_stacks) {
+    splash_mesh* m = nullptr;
+    int rc = splash_mesh_uv_sphere(n_slices, n_stacks, &m);
+    return SurfaceMesh::from_generator(rc, m);
+}
+inline SurfaceMesh icosphere(size_t n) {
+    splash_mesh* m = nullptr;
+    int rc = splash_mesh_icosphere(int(n), &m);
+    return SurfaceMesh::from_generator(rc, m);
+}
+inline SurfaceMesh icosphere_hex(size_t n) {
+    splash_mesh* m = nullptr;
+    int rc = splash_mesh_icosphere_hex(int(n), &m);
+    return SurfaceMesh::from_generator(rc, m);
+}
+inline SurfaceMesh quad_sphere(size_t n) {
+    splash_mesh* m = nullptr;
+    int rc = splash_mesh_quad_sphere(int(n), &m);
+    return SurfaceMesh::from_generator(rc, m);
+}
+
+// The flat keys of input/parameters.json the adiabatic constructor reads
+// (adiabatic.hpp:59-68).  Throws std::runtime_error when the file is missing
+// or a key cannot be parsed, as the reference's nlohmann::json::parse does.
+inline splash_params read_parameters_json(const std::string& path = "input/parameters.json") {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("parse error: cannot open " + path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    const std::string text = ss.str();
+    auto raw = [&](const std::string& key) -> std::string {
+        const std::string q = "\"" + key + "\"";
+        size_t p = text.find(q);
+        if (p == std::string::npos) throw std::runtime_error("parse error: key " + key + " missing in " + path);
+        p = text.find(':', p + q.size());
+        if (p == std::string::npos) throw std::runtime_error("parse error at key " + key);
+        size_t e = text.find_first_of(",}\n", p + 1);
+        std::string v = text.substr(p + 1, e == std::string::npos ? std::string::npos : e - p - 1);
+        const size_t a = v.find_first_not_of(" \t\r"), b = v.find_last_not_of(" \t\r");
+        return a == std::string::npos ? std::string() : v.substr(a, b - a + 1);
+    };
+    auto num = [&](const std::string& key) {
+        const std::string v = raw(key);
+        size_t used = 0;
+        double d = std::stod(v, &used);
+        if (used != v.size()) throw std::runtime_error("parse error at key " + key);
+        return d;
+    };
+    auto boolean = [&](const std::string& key) {
+        const std::string v = raw(key);
+        if (v == "true") return 1;
+        if (v == "false") return 0;
+        throw std::runtime_error("parse error at key " + key);
+    };
+    splash_params p;
+    splash_default_params(&p);
+    p.acc_rate = num("accretion_rate");
+    p.e_acc = num("e_acc");
+    p.omega_acc_abs = num("omega_acc_abs");
+    p.tilt_angle = num("tilt_angle");
+    p.acc_width = num("acc_width");
+    p.alpha = num("alpha");
+    p.friction_on = boolean("friction_on");
+    return p;
+}
+
+class MUSCL_gpu {
+public:
+    MUSCL_gpu(const MUSCL_gpu&) = delete;
+    MUSCL_gpu& operator=(const MUSCL_gpu&) = delete;
+    virtual ~MUSCL_gpu() { splash_destroy(ctx_); }
+
+    // MUSCL_base.hpp:97-225, including the per-step diagnostics line (:213-214)
+    void do_step(double dt0) {
+        check_rc(splash_do_step(ctx_, dt0));
+        if (verbose_) print_step_line();
+        ++steps_;
+    }
+    // n steps without per-step host work (HIP-graph replay); prints nothing
+    void do_steps(double dt0, int n) {
+        check_rc(splash_do_steps(ctx_, dt0, n));
+        steps_ += size_t(n);
+    }
+    double time() {
+        double t = 0;
+        check_rc(splash_time(ctx_, &t));
+        return t;
+    }
+    bool get_stop_check() {
+        int s = 0;
+        check_rc(splash_stop_check(ctx_, &s));
+        return s != 0;
+    }
+    void set_verbose(bool v) { verbose_ = v; }
+    int n_faces() const { return int(mesh_.n_faces()); }
+
+    std::vector<std::vector<double>> get_state() {
+        std::vector<double> flat(mesh_.n_faces() * size_t(dim_));
+        check_rc(splash_get_state(ctx_, flat.data()));
+        std::vector<std::vector<double>> U(mesh_.n_faces(), std::vector<double>(size_t(dim_)));
+        for (size_t i = 0; i < U.size(); ++i)
+            for (int k = 0; k < dim_; ++k) U[i][size_t(k)] = flat[i * size_t(dim_) + size_t(k)];
+        return U;
+    }
+
+    // MUSCL_geometry.hpp:1069-1117
+    void write_face_centers() {
+        std::ofstream out("results/face_centers.dat", std::ios::out);
+        for (size_t i = 0; i < fc_.size() / 3; ++i) {
+            const double n = norm3(&fc_[3 * i]);
+            out << std::setprecision(15) << fc_[3 * i] / n << " " << fc_[3 * i + 1] / n << " " << fc_[3 * i + 2] / n
+                << "\n";
+        }
+    }
+    void write_faces() {
+        std::ofstream out("results/faces.dat", std::ios::out);
+        const auto& fo = mesh_.face_offsets();
+        const auto& fv = mesh_.face_vertices();
+        for (size_t f = 0; f + 1 < fo.size(); ++f) {
+            for (int32_t k = fo[f]; k < fo[f + 1]; ++k) out << fv[size_t(k)] << " ";
+            out << "\n";
+        }
+    }
+    void write_vertices() {
+        std::ofstream out("results/vertices.dat", std::ios::out);
+        const auto& x = mesh_.vertices();
+        for (size_t v = 0; v < x.size() / 3; ++v) {
+            for (int c = 0; c < 3; ++c) out << std::setprecision(15) << x[3 * v + size_t(c)] << " ";
+            out << "\n";
+        }
+    }
+    void write_t_rho() {
+        auto U = get_state();
+        out_rho_ << time() << "  ";
+        for (auto& u : U) out_rho_ << u[0] << " ";
+        out_rho_ << "\n";
+    }
+
+protected:
+    MUSCL_gpu(const SurfaceMesh& mesh, const std::vector<std::vector<double>>& U_in, int dim, double gam,
+              double omega_ns, int solver, splash_params params)
+        : mesh_(mesh), dim_(dim), gam_(gam) {
+        if (U_in.size() != mesh.n_faces())
+            std::cout << "U_in is wrong size, U_in.size() should be equal to mesh.n_faces()" << std::endl;
+        std::vector<double> flat(mesh.n_faces() * size_t(dim), 1.0);
+        for (size_t i = 0; i < U_in.size() && i < mesh.n_faces(); ++i)
+            for (int k = 0; k < dim && size_t(k) < U_in[i].size(); ++k) flat[i * size_t(dim) + size_t(k)] = U_in[i][size_t(k)];
+        const splash_mesh_desc d = mesh.desc();
+        check_rc(splash_create(&d, flat.data(), dim, solver, gam, omega_ns, &params, &ctx_));
+        fc_.resize(3 * mesh.n_faces());
+        area_.resize(mesh.n_faces());
+        check_rc(splash_face_geometry(ctx_, fc_.data(), area_.data()));
+        out_rho_.open("results/rho.dat", std::ios::out | std::ios::trunc);
+    }
+
+    static double norm3(const double* p) { return std::sqrt(((0.0 + p[0] * p[0]) + p[1] * p[1]) + p[2] * p[2]); }
+
+    void print_step_line() {
+        double d[5];
+        check_rc(splash_diagnostics(ctx_, d));
+        if (steps_ == 0) {
+            rho_full_ = d[0];
+            E_full_ = d[4];
+        }
+        std::cout << "t= " << time() << " mass_relative_gain=" << d[0] / rho_full_ - 1
+                  << " (l/mass)_total_norm= " << std::sqrt(d[1] * d[1] + d[2] * d[2] + d[3] * d[3]) << " l_total = ("
+                  << d[1] << "," << d[2] << "," << d[3] << ")" << "\n";
+    }
+
+    splash_ctx* ctx_ = nullptr;
+    SurfaceMesh mesh_;
+    int dim_;
+    double gam_;
+    size_t steps_ = 0;
+    double rho_full_ = 0, E_full_ = 0;
+    bool verbose_ = true;
+    std::vector<double> fc_, area_;
+    std::ofstream out_rho_;
+};
+
+// isothermal physics (isothermal.hpp): writers of p and omega_z per
+// isothermal.hpp:60-70, :113-133
+class isothermal_gpu : public MUSCL_gpu {
+public:
+    void write_t_p() {
+        auto U = get_state();
+        out_p_ << time() << "  ";
+        for (auto& u : U) out_p_ << 1.0 * 1.0 * u[0] << " ";
+        out_p_ << "\n";
+    }
+
+protected:
+    isothermal_gpu(const SurfaceMesh& mesh, const std::vector<std::vector<double>>& U_in, int dim, double gam,
+                   int solver, const splash_params& p)
+        : MUSCL_gpu(mesh, U_in, dim, gam, 0.0, solver, p) {
+        out_p_.open("results/p.dat", std::ios::out | std::ios::trunc);
+        std::ofstream("results/curl.dat", std::ios::out | std::ios::trunc);
+        std::ofstream("results/omega.dat", std::ios::out | std::ios::trunc);
+    }
+    std::ofstream out_p_;
+};
+
+// adiabatic physics (adiabatic.hpp): writers, light curves and final state
+class adiabatic_gpu : public MUSCL_gpu {
+public:
+    // adiabatic.hpp:113-133: pressure at the face centre
+    void write_t_p() {
+        auto U = get_state();
+        out_p_ << time() << "  ";
+        for (size_t i = 0; i < U.size(); ++i) out_p_ << pressure_at_centre(U[i], i) << " ";
+        out_p_ << "\n";
+    }
+    // adiabatic.hpp:179-193
+    void write_t_L() {
+        auto U = get_state();
+        for (int c = 0; c < 3; ++c) {
+            out_l_[c] << time() << "  ";
+            for (auto& u : U) out_l_[c] << u[size_t(c) + 1] << " ";
+            out_l_[c] << "\n";
+        }
+    }
+    // adiabatic.hpp:195-226
+    void write_t_omega_z() {
+        auto U = get_state();
+        out_omega_ << time() << "  ";
+        for (size_t i = 0; i < U.size(); ++i) {
+            double r[3];
+            unit_centre(i, r);
+            const double theta = std::acos(fc_[3 * i + 2] / norm3(&fc_[3 * i]));
+            double v[3];
+            velocity(U[i], r, v);
+            const double rxv_z = r[0] * v[1] - r[1] * v[0];
+            double om = rxv_z / (std::sin(theta) * std::sin(theta));
+            if (std::isnan(om) || std::isinf(om) || std::abs(om) > 1e4) om = 0;
+            out_omega_ << om << " ";
+        }
+        out_omega_ << "\n";
+    }
+    // adiabatic.hpp:335-348
+    void write_final_state() {
+        auto U = get_state();
+        std::ofstream out("results/final_state.dat");
+        for (auto& u : U) {
+            for (int j = 0; j < dim_; ++j) out << u[size_t(j)] << " ";
+            out << "\n";
+        }
+    }
+    // adiabatic.hpp:228-333: pressure-weighted projected flux towards four observers
+    std::vector<double> get_light_curves() {
+        auto U = get_state();
+        const double d = 9.4 * 3 * 1e19 / 15000, s = std::sqrt(9.4 * 3 * 1e19 / 15000);
+        const double obs[4][3] = {{d, 0, 0}, {s, 0, s}, {0, 0, d}, {0, 0, -d}};
+        double tot[4] = {0, 0, 0, 0};
+        for (size_t i = 0; i < U.size(); ++i) {
+            double r[3];
+            unit_centre(i, r);
+            const double phi = std::atan2(r[1], r[0]);
+            const double theta = std::acos(fc_[3 * i + 2] / norm3(&fc_[3 * i]));
+            const double PI = pressure_at_centre(U[i], i);
+            auto cos_to = [&](int k) {
+                const double dv = obs[k][0] * r[0] + obs[k][1] * r[1] + obs[k][2] * r[2];
+                return std::abs(dv) / norm3(obs[k]);
+            };
+            if (phi < M_PI / 2 && phi > -M_PI / 2) tot[0] += PI * cos_to(0) * area_[i];
+            if (theta < M_PI / 2) tot[2] += PI * cos_to(2) * area_[i];
+            if (obs[1][0] * r[0] + obs[1][1] * r[1] + obs[1][2] * r[2] > 0) tot[1] += PI * cos_to(1) * area_[i];
+            if (theta > M_PI / 2) tot[3] += PI * cos_to(3) * area_[i];
+        }
+        return {tot[0], tot[1], tot[2], tot[3]};
+    }
+
+protected:
+    adiabatic_gpu(const SurfaceMesh& mesh, const std::vector<std::vector<double>>& U_in, int dim, double gam,
+                  double omega_ns, int solver, const splash_params& p)
+        : MUSCL_gpu(mesh, U_in, dim, gam, omega_ns, solver, p) {
+        if (dim != 5) std::cout << "check dim \n";
+        std::ofstream("results/curl.dat", std::ios::out | std::ios::trunc);
+        out_p_.open("results/p.dat", std::ios::out | std::ios::trunc);
+        out_omega_.open("results/omega.dat", std::ios::out | std::ios::trunc);
+        const char* adrs[3] = {"results/Lx.dat", "results/Ly.dat", "results/Lz.dat"};
+        for (int c = 0; c < 3; ++c) out_l_[c].open(adrs[c], std::ios::out | std::ios::trunc);
+    }
+    static splash_params params_from_json(bool accretion_on) {
+        splash_params p = read_parameters_json();
+        p.accretion_on = accretion_on ? 1 : 0;
+        return p;
+    }
+    void unit_centre(size_t i, double* r) const {
+        const double n = norm3(&fc_[3 * i]);
+        for (int c = 0; c < 3; ++c) r[c] = fc_[3 * i + size_t(c)] / n;
+    }
+    static void velocity(const std::vector<double>& u, const double* r, double* v) {
+        v[0] = (r[1] * u[3] - r[2] * u[2]) / (-u[0]);
+        v[1] = (r[2] * u[1] - r[0] * u[3]) / (-u[0]);
+        v[2] = (r[0] * u[2] - r[1] * u[1]) / (-u[0]);
+    }
+    double pressure_at_centre(const std::vector<double>& u, size_t i) const {
+        double r[3], v[3];
+        unit_centre(i, r);
+        velocity(u, r, v);
+        const double n = norm3(v);
+        return std::max(1e-16, (u[4] - u[0] * (n * n) / 2) * (gam_ - 1));
+    }
+    std::ofstream out_p_, out_omega_, out_l_[3];
+};
+
+class MUSCL_HLLE : public isothermal_gpu {
+public:
+    MUSCL_HLLE(SurfaceMesh mesh, std::vector<std::vector<double>> U_in, int dim, double gam, size_t threads = 1,
+               int precision = SPLASH_FP64)
+        : isothermal_gpu(mesh, U_in, dim, gam, SPLASH_HLLE, with_precision(precision)) {
+        (void)threads;
+    }
+
+private:
+    static splash_params with_precision(int precision) {
+        splash_params p;
+        splash_default_params(&p);
+        p.precision = precision;
+        return p;
+    }
+};
+
+#define SPLASH_ADIABATIC_SOLVER(NAME, ID)                                                                          \
+    class NAME : public adiabatic_gpu {                                                                            \
+    public:                                                                                                        \
+        NAME(SurfaceMesh mesh, std::vector<std::vector<double>> U_in, int dim, double gam, double omega_ns,       \
+             bool accretion_on = false, size_t threads = 1)                                                       \
+            : adiabatic_gpu(mesh, U_in, dim, gam, omega_ns, ID, params_from_json(accretion_on)) {                 \
+            (void)threads;                                                                                         \
+        }                                                                                                          \
+        /* explicit parameters instead of input/parameters.json (no file I/O for parameters) */                   \
+        NAME(SurfaceMesh mesh, std::vector<std::vector<double>> U_in, int dim, double gam, double omega_ns,       \
+             const splash_params& params)                                                                          \
+            : adiabatic_gpu(mesh, U_in, dim, gam, omega_ns, ID, params) {}                                         \
+    };
+
+SPLASH_ADIABATIC_SOLVER(MUSCL_HLLE_p, SPLASH_HLLE_P)
+SPLASH_ADIABATIC_SOLVER(MUSCL_HLLC, SPLASH_HLLC)
+SPLASH_ADIABATIC_SOLVER(MUSCL_HLLCplus, SPLASH_HLLCPLUS)
+
+#undef SPLASH_ADIABATIC_SOLVER
+
+}  // namespace splash
