@@ -37,8 +37,8 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s) {
 }
 
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s) {
-    if (a.phys.dim == 5) k_finalize_dt<5><<<1, kBlock, 0, s>>>(a.geom, a.phys, a.sc, dt0);
-    else k_finalize_dt<4><<<1, kBlock, 0, s>>>(a.geom, a.phys, a.sc, dt0);
+    if (a.phys.dim == 5) k_finalize_dt<5><<<1, kFinalizeThreads, 0, s>>>(a.geom, a.phys, a.sc, dt0);
+    else k_finalize_dt<4><<<1, kFinalizeThreads, 0, s>>>(a.geom, a.phys, a.sc, dt0);
 }
 
 

@@ -123,14 +123,17 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     }
     __syncthreads();
 
-    if (t < n_own) {
-        const int h = hbase + t;
+    // one flux per edge: the block evaluates its task half-edges from the
+    // edge's lower-index side and writes +F / -F into both own slots, so the
+    // two faces of an edge exchange exactly opposite fluxes (mass, momentum
+    // and energy conserved to round-off)
+    const int toff = g.task_off[b];
+    if (t < g.task_off[b + 1] - toff) {
+        const int h = g.task_he[toff + t];
+        const int sh = h - hbase;
         const int2 tw = reinterpret_cast<const int2*>(g.twin_slot)[h];
-        // every edge is evaluated from its lower-index half-edge; the other
-        // side takes the negated flux, so both faces exchange exactly opposite
-        // values and mass, momentum and energy are conserved to round-off
-        const bool own_side = h < tw.x;
-        const int he = own_side ? h : tw.x;
+        const bool low = h < tw.x;
+        const int he = low ? h : tw.x;
         const float4 e4 = reinterpret_cast<const float4*>(g.ec32)[he];
         const float4 n4 = reinterpret_cast<const float4*>(g.en32)[he];
         Edge<float> e;
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         e.ecn = e4.w;
         e.en = v3(n4.x, n4.y, n4.z);
         float uL[DIM], uR[DIM], F[DIM];
-        const int sl = own_side ? t : tw.y, sr = own_side ? tw.y : t;
+        const int sl = low ? sh : tw.y, sr = low ? tw.y : sh;
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
             uL[k] = sUP[sl * DIM + k];
@@ -146,8 +149,16 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         }
         const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
         if (err) raise_stop(sc, err, h);
+        if (low) {
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) sF[t * DIM + k] = own_side ? F[k] : -F[k];
+            for (int k = 0; k < DIM; ++k) sF[sh * DIM + k] = F[k];
+            if (tw.y < n_own)
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) sF[tw.y * DIM + k] = -F[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) sF[sh * DIM + k] = -F[k];
+        }
     }
     __syncthreads();
 
