@@ -97,8 +97,6 @@ void Engine::upload() {
         halo_off_ = std::move(L.halo_off);
         halo_he_ = std::move(L.halo_he);
         twin_slot_ = std::move(L.twin_slot);
-        task_off_ = std::move(L.task_off);
-        task_he_ = std::move(L.task_he);
     }
     auto dev = [&](size_t bytes, const void* host) -> void* {
         void* p = nullptr;
@@ -203,14 +201,12 @@ void Engine::upload() {
         d.hsc32 = static_cast<const float*>(dev(sizeof(float) * h32.size(), h32.data()));
         d.ec32 = static_cast<const float*>(dev(sizeof(float) * e32.size(), e32.data()));
         d.en32 = static_cast<const float*>(dev(sizeof(float) * n32.size(), n32.data()));
-        d.task_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * task_off_.size(), task_off_.data()));
-        d.task_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * task_he_.size(), task_he_.data()));
+
         const size_t row = dim_ == 5 ? 8 : 4;
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
         d.beta32 = d.hsc32 = d.ec32 = d.en32 = nullptr;
-        d.task_off = d.task_he = nullptr;
         args_.W32 = args_.W32b = nullptr;
     }
     d.n_blocks = int32_t(blk_face_.size()) - 1;

@@ -64,8 +64,6 @@ struct GeomPtrs {
     const float* hsc32;       // [nh][4]: H+, H-, BM_dist, arc/area
     const float* ec32;        // [nh][4]: unit edge midpoint xyz, |ec|
     const float* en32;        // [nh][4]: edge normal xyz, chord length
-    const int32_t* task_off;  // [n_blocks+1] one-sided flux tasks of each block
-    const int32_t* task_he;
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
 };

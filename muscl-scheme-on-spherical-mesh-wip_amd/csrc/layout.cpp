@@ -102,16 +102,6 @@ Layout build_layout(const Geometry& g, int32_t slots) {
         }
         L.halo_off.push_back(int32_t(L.halo_he.size()));
     }
-    L.task_off.assign(1, 0);
-    for (int32_t b = 0; b < nb; ++b) {
-        const int32_t h0 = L.foff[L.blk_face[b]], h1 = L.foff[L.blk_face[b + 1]];
-        for (int32_t hn = h0; hn < h1; ++hn) {
-            const int32_t tw = L.twin_slot[2 * size_t(hn)];
-            const bool twin_own = tw >= h0 && tw < h1;
-            if (hn < tw || !twin_own) L.task_he.push_back(hn);
-        }
-        L.task_off.push_back(int32_t(L.task_he.size()));
-    }
     return L;
 }
 
@@ -138,18 +128,18 @@ void validate_layout(const Geometry& g, const Layout& L) {
             const int32_t he = L.halo_he[q];
             require(he >= 0 && he < nh && (he < h0 || he >= h1), "halo half-edges are outside the block");
         }
-        // every own half-edge receives exactly one flux from the block's tasks
+        // one-sided fluxes of the mixed kernel: an own half-edge evaluates its
+        // edge when it is the lower-index side or its twin is in the halo, and
+        // also fills its twin's slot when that twin is own; every own slot must
+        // be written exactly once
         std::vector<int32_t> filled(size_t(n_own), 0);
-        require(int32_t(L.task_off.size()) == nb + 1, "task offsets");
-        for (int32_t q = L.task_off[b]; q < L.task_off[b + 1]; ++q) {
-            const int32_t he = L.task_he[q];
-            require(he >= h0 && he < h1, "tasks are own half-edges");
-            filled[size_t(he - h0)]++;
-            const int32_t tw = L.twin_slot[2 * size_t(he)];
-            if (he < tw && tw >= h0 && tw < h1) filled[size_t(tw - h0)]++;
+        for (int32_t hn = h0; hn < h1; ++hn) {
+            const int32_t tw = L.twin_slot[2 * size_t(hn)];
+            const bool twin_own = tw >= h0 && tw < h1;
+            if (hn < tw || !twin_own) filled[size_t(hn - h0)]++;
+            if (hn < tw && twin_own) filled[size_t(tw - h0)]++;
         }
         for (int32_t q = 0; q < n_own; ++q) require(filled[size_t(q)] == 1, "each own flux written once");
-        require(L.task_off[b + 1] - L.task_off[b] <= L.slots, "tasks fit the block");
         for (int32_t hn = h0; hn < h1; ++hn) {
             const int32_t tw = L.twin_slot[2 * size_t(hn)], slot = L.twin_slot[2 * size_t(hn) + 1];
             require(tw >= 0 && tw < nh && L.twin_slot[2 * size_t(tw)] == hn, "twin is an involution");

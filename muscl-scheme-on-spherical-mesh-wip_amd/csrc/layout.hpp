@@ -22,11 +22,6 @@ struct Layout {
     std::vector<int32_t> halo_off;     // [n_blocks+1]
     std::vector<int32_t> halo_he;      // internal half-edge ids
     std::vector<int32_t> twin_slot;    // [nh][2]: internal twin, its slot in the owner's block
-    // one-sided flux tasks (mixed path): per block, the own half-edges whose
-    // edge flux the block evaluates -- the lower-index side of every edge plus
-    // the own side of every edge whose lower-index side is in the halo
-    std::vector<int32_t> task_off;     // [n_blocks+1]
-    std::vector<int32_t> task_he;
     int32_t n_blocks() const { return int32_t(blk_face.size()) - 1; }
 };
 

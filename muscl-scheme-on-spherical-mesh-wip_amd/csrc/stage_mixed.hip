@@ -57,15 +57,16 @@ __device__ __forceinline__ void store_w32(float* __restrict__ W32, int i, const 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433)
 template <int DIM>
 __device__ __forceinline__ void reconstruct32(const GeomPtrs& g, const PhysParams& P, float dt,
-                                              const float* __restrict__ W32, int h, float* out) {
+                                              const float* __restrict__ W32, int h, float* out, float4& e4,
+                                              float4& n4) {
     const int fv = g.he_fv[h];
     const int i = fv & 0x07ffffff;
     const int valence = fv >> 27;
     const int4 st = reinterpret_cast<const int4*>(g.sten)[h];
     const float4 bt = reinterpret_cast<const float4*>(g.beta32)[h];
     const float4 hs = reinterpret_cast<const float4*>(g.hsc32)[h];  // 1/H+, 1/H-, BM, arc/area
-    const float4 e4 = reinterpret_cast<const float4*>(g.ec32)[h];
-    const float4 n4 = reinterpret_cast<const float4*>(g.en32)[h];
+    e4 = reinterpret_cast<const float4*>(g.ec32)[h];
+    n4 = reinterpret_cast<const float4*>(g.en32)[h];
     float wi[DIM], a[DIM], b[DIM], c[DIM], d[DIM];
     load_w32<DIM>(W32, i, wi);
     load_w32<DIM>(W32, st.x, a);
@@ -101,13 +102,45 @@ __device__ __forceinline__ void reconstruct32(const GeomPtrs& g, const PhysParam
     }
 }
 
+// pressure_fc, find_v_max and extra_dt_constr terms of one E-form state
+// (as face_terms) with one FP64 reciprocal of the density instead of one
+// division per use: the p-form feeds the FP32 stage input, and the dt limits
+// stay FP64 within an ulp of the exact quotients.
+template <int DIM>
+__device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const double* u, V3<double> fcn, double* w,
+                                                 double& vmax, double& dtx) {
+    const double inv = 1.0 / u[0];
+    const V3<double> rl = cross(fcn, v3(u[1], u[2], u[3]));
+    const double vv2 = dot(rl, rl) * (inv * inv);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) w[k] = u[k];
+    double cs;
+    if constexpr (DIM == 5) {
+        const double p = smax(1e-16, (u[4] - u[0] * vv2 * 0.5) * (P.gam - 1));
+        w[4] = p;
+        cs = sqrt(P.gam * p * inv);
+        const double gam3d = 1 / (2 - P.gam);
+        const double g_eff = kGM - vv2;
+        const double rho = gam3d / (2 * gam3d - 1) * g_eff * u[0] * u[0] / p;
+        const V3<double> vel = rl * (-inv);
+        const V3<double> vel0 = cross(v3(0.0, 0.0, P.omega_ns), fcn);
+        const double cand = 0.01 * u[0] / (P.alpha * rho * norm(vel - vel0));
+        if (dtx > cand) dtx = cand;
+    } else {
+        cs = 1.0;
+    }
+    const double vn = sqrt(vv2);
+    if (vn > vmax) vmax = vn;
+    if (cs > vmax) vmax = cs;
+}
+
 template <int DIM, int SOLVER, int STAGE>
 __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Scalars* __restrict__ sc,
                                                     double* __restrict__ U, const float* __restrict__ W32,
                                                     float* __restrict__ W32_out, const double* __restrict__ W,
                                                     double* __restrict__ W_out) {
     __shared__ float sUP[kBlock * DIM];
-    __shared__ float sF[kBlock * DIM];
+    __shared__ float sF[kBlock * DIM];  // edge flux times chord length, per own slot
     const int b = xcd_block();
     const int f0 = g.blk_face[b], f1 = g.blk_face[b + 1];
     const int hbase = g.foff[f0];
@@ -117,56 +150,69 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     const int t = threadIdx.x;
     const double dt = sc->dt;
 
-    if (t < n_slots) {
-        const int h = t < n_own ? hbase + t : g.halo_he[hoff + t - n_own];
-        reconstruct32<DIM>(g, P, float(dt), W32, h, &sUP[t * DIM]);
-    }
-    __syncthreads();
-
-    // one flux per edge: the block evaluates its task half-edges from the
-    // edge's lower-index side and writes +F / -F into both own slots, so the
-    // two faces of an edge exchange exactly opposite fluxes (mass, momentum
-    // and energy conserved to round-off)
-    const int toff = g.task_off[b];
-    if (t < g.task_off[b + 1] - toff) {
-        const int h = g.task_he[toff + t];
-        const int sh = h - hbase;
-        const int2 tw = reinterpret_cast<const int2*>(g.twin_slot)[h];
-        const bool low = h < tw.x;
-        const int he = low ? h : tw.x;
-        const float4 e4 = reinterpret_cast<const float4*>(g.ec32)[he];
-        const float4 n4 = reinterpret_cast<const float4*>(g.en32)[he];
-        Edge<float> e;
-        e.ec = v3(e4.x, e4.y, e4.z);
-        e.ecn = e4.w;
-        e.en = v3(n4.x, n4.y, n4.z);
-        float uL[DIM], uR[DIM], F[DIM];
-        const int sl = low ? sh : tw.y, sr = low ? tw.y : sh;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-            uL[k] = sUP[sl * DIM + k];
-            uR[k] = sUP[sr * DIM + k];
-        }
-        const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
-        if (err) raise_stop(sc, err, h);
-        if (low) {
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) sF[sh * DIM + k] = F[k];
-            if (tw.y < n_own)
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) sF[tw.y * DIM + k] = -F[k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) sF[sh * DIM + k] = -F[k];
-        }
-    }
-    __syncthreads();
-
+    // face data of phase 3, requested before the stencil gathers
     const int i = f0 + t;
+    const bool face = i < f1;
+    double u0[DIM], area = 1;
+    V3<double> fcn = v3(0.0, 0.0, 0.0);
+    int hb = 0, he = 0;
+    if (face) {
+        hb = g.foff[i];
+        he = g.foff[i + 1];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u0[k] = U[size_t(i) * DIM + k];
+        const double4 c4 = reinterpret_cast<const double4*>(g.fcn)[i];
+        fcn = v3(c4.x, c4.y, c4.z);
+        area = g.area[i];
+    }
+
+    // phase 1: reconstruct own and halo half-edges into LDS
+    const bool own = t < n_own;
+    int2 tw = make_int2(0, 0);
+    float4 e4, n4;
+    if (own) tw = reinterpret_cast<const int2*>(g.twin_slot)[hbase + t];
+    if (t < n_slots) {
+        const int h = own ? hbase + t : g.halo_he[hoff + t - n_own];
+        reconstruct32<DIM>(g, P, float(dt), W32, h, &sUP[t * DIM], e4, n4);
+    }
+    __syncthreads();
+
+    // phase 2: one flux per edge, evaluated from the edge's lower-index
+    // half-edge (the twin side has the same midpoint and the negated normal)
+    // and written as +F*chord / -F*chord, so the two faces of an edge
+    // exchange exactly opposite fluxes
+    if (own) {
+        const int h = hbase + t;
+        const bool low = h < tw.x;
+        if (low || tw.y >= n_own) {
+            Edge<float> e;
+            e.ec = v3(e4.x, e4.y, e4.z);
+            e.ecn = e4.w;
+            e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
+            float uL[DIM], uR[DIM], F[DIM];
+            const int sl = low ? t : tw.y, sr = low ? tw.y : t;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+                uL[k] = sUP[sl * DIM + k];
+                uR[k] = sUP[sr * DIM + k];
+            }
+            const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
+            if (err) raise_stop(sc, err, h);
+            const float chord = low ? n4.w : -n4.w;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) sF[t * DIM + k] = F[k] * chord;
+            if (low && tw.y < n_own)
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) sF[tw.y * DIM + k] = -(F[k] * chord);
+        }
+    }
+    __syncthreads();
+
+    // phase 3: res2d (compensated FP64 sum of the edge terms), source, RK
+    // combine and density floor for one face per thread
     double vmax = 1e-8, dtx = 1e20;
-    if (i < f1) {
+    if (face) {
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
-        const V3<double> fcn = v3(g.fcn[4 * size_t(i)], g.fcn[4 * size_t(i) + 1], g.fcn[4 * size_t(i) + 2]);
         double res[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) res[k] = 0;
@@ -180,31 +226,29 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
                 for (int k = 0; k < 5; ++k) res[k] = dt_here * src[k];
             }
         }
-        const double rd_dt_A = P.round_diff * dt_here / g.area[i];
-        const int hb = g.foff[i], he = g.foff[i + 1];
         double sum[DIM], comp[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) sum[k] = comp[k] = 0;
         bool nan_flux = false;
         for (int hh = hb; hh < he; ++hh) {
-            const double coef = rd_dt_A * double(reinterpret_cast<const float4*>(g.en32)[hh].w);
             const float* Fh = &sF[(hh - hbase) * DIM];
 #pragma unroll
             for (int k = 0; k < DIM; ++k) {
                 nan_flux |= __builtin_isnan(Fh[k]);
-                const double x = coef * double(Fh[k]);
+                const double x = double(Fh[k]);
                 const double s = sum[k] + x;
                 comp[k] += (fabs(sum[k]) >= fabs(x)) ? (sum[k] - s) + x : (x - s) + sum[k];
                 sum[k] = s;
             }
         }
         if (nan_flux) raise_stop(sc, splash::kStopNaNFlux, i);
+        const double coef = P.round_diff * dt_here / area;
         double un[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[k] = (res[k] - (sum[k] + comp[k])) + U[size_t(i) * DIM + k];
+        for (int k = 0; k < DIM; ++k) un[k] = (res[k] - coef * (sum[k] + comp[k])) + u0[k];
         if (un[0] < 1e-8) un[0] = 1e-8;
         double wn[DIM];
-        face_terms<DIM>(P, un, fcn, wn, vmax, dtx);
+        face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
         store_w32<DIM>(W32_out, i, wn);
         if (P.accretion_on) {
 #pragma unroll
