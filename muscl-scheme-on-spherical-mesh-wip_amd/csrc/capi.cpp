@@ -292,7 +292,7 @@ int splash_geometry_get_view(const splash_geometry* g, splash_geometry_view* v) 
 
 void splash_geometry_free(splash_geometry* g) { delete g; }
 
-int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[4]) {
+int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
     if (!out) return fail(SPLASH_EINVAL, "null output");
     return guarded([&] {
         const splash::Geometry g = splash::build_geometry(from_desc(mesh), 0);
@@ -307,6 +307,9 @@ int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[4]) {
         out[1] = nonempty;
         out[2] = int32_t(L.halo_he.size());
         out[3] = max_halo;
+        int32_t max_fset = 0;
+        for (int32_t b = 0; b < L.n_blocks(); ++b) max_fset = std::max(max_fset, L.fset_off[b + 1] - L.fset_off[b]);
+        out[4] = max_fset;
         return SPLASH_OK;
     });
 }

@@ -97,6 +97,9 @@ void Engine::upload() {
         halo_off_ = std::move(L.halo_off);
         halo_he_ = std::move(L.halo_he);
         twin_slot_ = std::move(L.twin_slot);
+        fset_off_ = std::move(L.fset_off);
+        fset_face_ = std::move(L.fset_face);
+        ext_loc_ = std::move(L.ext_loc);
     }
     auto dev = [&](size_t bytes, const void* host) -> void* {
         void* p = nullptr;
@@ -183,30 +186,39 @@ void Engine::upload() {
     d.halo_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_off_.size(), halo_off_.data()));
     d.halo_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_he_.size(), halo_he_.data()));
     if (precision_ == kMixed) {
-        // FP32 records of the mixed kernel: beta, {1/H+, 1/H-, BM, arc/area},
-        // {ec, |ec|}, {en, chord}
-        std::vector<float> b32(4 * size_t(nh)), h32(4 * size_t(nh)), e32(4 * size_t(nh)), n32(4 * size_t(nh));
-        for (size_t q = 0; q < 4 * size_t(nh); ++q) {
-            b32[q] = float(beta[q]);
-            e32[q] = float(ec[q]);
-            n32[q] = float(en[q]);
+        // FP32 rows of the mixed kernel per extended slot row: beta,
+        // {1/H+, 1/H-, BM, arc/area}, {ec, |ec|}, {en, chord}
+        const size_t n_ext = ext_loc_.size() / 4;
+        std::vector<float> b32(4 * n_ext), h32(4 * n_ext), e32(4 * n_ext), n32(4 * n_ext);
+        for (size_t e = 0; e < n_ext; ++e) {
+            const size_t hn = size_t(ext_loc_[4 * e + 3]);
+            for (int c = 0; c < 4; ++c) {
+                b32[4 * e + c] = float(beta[4 * hn + c]);
+                e32[4 * e + c] = float(ec[4 * hn + c]);
+                n32[4 * e + c] = float(en[4 * hn + c]);
+            }
+            h32[4 * e + 0] = float(1.0 / hsc[4 * hn + 0]);
+            h32[4 * e + 1] = float(1.0 / hsc[4 * hn + 1]);
+            h32[4 * e + 2] = float(hsc[4 * hn + 2]);
+            h32[4 * e + 3] = float(hsc[4 * hn + 3]);
         }
-        for (int32_t hn = 0; hn < nh; ++hn) {
-            h32[4 * size_t(hn) + 0] = float(1.0 / hsc[4 * size_t(hn) + 0]);
-            h32[4 * size_t(hn) + 1] = float(1.0 / hsc[4 * size_t(hn) + 1]);
-            h32[4 * size_t(hn) + 2] = float(hsc[4 * size_t(hn) + 2]);
-            h32[4 * size_t(hn) + 3] = float(hsc[4 * size_t(hn) + 3]);
-        }
+        d.ext_loc = static_cast<const int32_t*>(dev(sizeof(int32_t) * ext_loc_.size(), ext_loc_.data()));
         d.beta32 = static_cast<const float*>(dev(sizeof(float) * b32.size(), b32.data()));
         d.hsc32 = static_cast<const float*>(dev(sizeof(float) * h32.size(), h32.data()));
         d.ec32 = static_cast<const float*>(dev(sizeof(float) * e32.size(), e32.data()));
         d.en32 = static_cast<const float*>(dev(sizeof(float) * n32.size(), n32.data()));
-
+        d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off_.size(), fset_off_.data()));
+        d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face_.size(), fset_face_.data()));
+        int32_t mx = 1;
+        for (size_t q = 0; q + 1 < fset_off_.size(); ++q) mx = std::max(mx, fset_off_[q + 1] - fset_off_[q]);
+        d.fset_max = mx;
         const size_t row = dim_ == 5 ? 8 : 4;
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
+        d.ext_loc = d.fset_off = d.fset_face = nullptr;
         d.beta32 = d.hsc32 = d.ec32 = d.en32 = nullptr;
+        d.fset_max = 0;
         args_.W32 = args_.W32b = nullptr;
     }
     d.n_blocks = int32_t(blk_face_.size()) - 1;

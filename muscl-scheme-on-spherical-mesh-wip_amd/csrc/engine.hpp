@@ -84,7 +84,7 @@ private:
     std::vector<int32_t> perm_, iperm_;      // internal -> mesh face, mesh -> internal
     std::vector<int32_t> foff_;              // internal face offsets
     std::vector<int32_t> he_old_, he_new_;   // internal -> mesh half-edge and back
-    std::vector<int32_t> blk_face_, halo_off_, halo_he_, twin_slot_;
+    std::vector<int32_t> blk_face_, halo_off_, halo_he_, twin_slot_, fset_off_, fset_face_, ext_loc_;
 
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;

@@ -58,12 +58,17 @@ struct GeomPtrs {
     const int32_t* blk_face;  // [n_blocks+1] face range of each stage block
     const int32_t* halo_off;  // [n_blocks+1] range of each block's halo in halo_he
     const int32_t* halo_he;   // outside twins of each block's boundary half-edges
-    // FP32 copies for the mixed-precision kernel (null in FP64 runs); each
-    // holds exactly the value the FP64 table converts to at load time
-    const float* beta32;      // [nh][4]
-    const float* hsc32;       // [nh][4]: H+, H-, BM_dist, arc/area
-    const float* ec32;        // [nh][4]: unit edge midpoint xyz, |ec|
-    const float* en32;        // [nh][4]: edge normal xyz, chord length
+    // mixed-precision kernel only (null in FP64 runs): FP32 rows per
+    // extended slot row (block b's own then halo half-edges start at
+    // foff[f0] + halo_off[b]) and the LDS face sets
+    const int32_t* ext_loc;   // [n_ext][4]: owner local | valence << 16, plus0 | plus1 << 16, minus0 | minus1 << 16, he
+    const float* beta32;      // [n_ext][4]
+    const float* hsc32;       // [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
+    const float* ec32;        // [n_ext][4]: unit edge midpoint xyz, |ec|
+    const float* en32;        // [n_ext][4]: edge normal xyz, chord length
+    const int32_t* fset_off;  // [n_blocks+1]
+    const int32_t* fset_face; // internal face ids, each block's own faces first
+    int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
 };

@@ -102,6 +102,41 @@ Layout build_layout(const Geometry& g, int32_t slots) {
         }
         L.halo_off.push_back(int32_t(L.halo_he.size()));
     }
+
+    // face sets and extended slot rows
+    const int32_t n_ext = nh + int32_t(L.halo_he.size());
+    L.ext_loc.assign(4 * size_t(n_ext), 0);
+    L.fset_off.assign(1, 0);
+    std::vector<int32_t> local(nf, -1);
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+        const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
+        const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
+        const size_t first = L.fset_face.size();
+        auto loc = [&](int32_t f) {
+            if (local[f] < 0) {
+                local[f] = int32_t(L.fset_face.size() - first);
+                L.fset_face.push_back(f);
+            }
+            return local[f];
+        };
+        for (int32_t f = f0; f < f1; ++f) loc(f);
+        for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
+            const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+            const int32_t h = L.he_old[he];
+            const int32_t owner = L.iperm[g.he_face[h]];
+            const int32_t val = g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]];
+            int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
+            row[0] = loc(owner) | (val << 16);
+            row[1] = loc(L.iperm[g.ff_plus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_plus[2 * size_t(h) + 1]]) << 16);
+            row[2] = loc(L.iperm[g.ff_minus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_minus[2 * size_t(h) + 1]]) << 16);
+            row[3] = he;
+        }
+        if (int32_t(L.fset_face.size() - first) > Layout::kFaceSetMax)
+            throw std::runtime_error("a block's stencil face set exceeds the LDS face slots");
+        for (size_t q = first; q < L.fset_face.size(); ++q) local[L.fset_face[q]] = -1;
+        L.fset_off.push_back(int32_t(L.fset_face.size()));
+    }
     return L;
 }
 
@@ -118,6 +153,7 @@ void validate_layout(const Geometry& g, const Layout& L) {
     for (int32_t hn = 0; hn < nh; ++hn) require(L.he_new[L.he_old[hn]] == hn, "half-edge maps are inverse");
     require(nb % 8 == 0 && L.blk_face[0] == 0 && L.blk_face[nb] == nf, "blocks cover the faces");
     require(int32_t(L.halo_off.size()) == nb + 1 && L.halo_off[nb] == int32_t(L.halo_he.size()), "halo offsets");
+    require(int32_t(L.fset_off.size()) == nb + 1 && L.n_ext() == nh + int32_t(L.halo_he.size()), "face sets");
     for (int32_t b = 0; b < nb; ++b) {
         const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
         require(f0 <= f1 && f1 - f0 <= L.slots, "faces per block <= threads");
@@ -140,6 +176,26 @@ void validate_layout(const Geometry& g, const Layout& L) {
             if (hn < tw && twin_own) filled[size_t(tw - h0)]++;
         }
         for (int32_t q = 0; q < n_own; ++q) require(filled[size_t(q)] == 1, "each own flux written once");
+        // face set: own faces first, every local index in range and pointing
+        // at the face the geometry names
+        const int32_t fs0 = L.fset_off[b], n_fs = L.fset_off[b + 1] - fs0;
+        require(n_fs >= f1 - f0 && n_fs <= Layout::kFaceSetMax, "face set size");
+        for (int32_t f = f0; f < f1; ++f) require(L.fset_face[fs0 + f - f0] == f, "own faces lead the face set");
+        for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
+            const int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
+            const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+            require(row[3] == he, "extended row names its half-edge");
+            const int32_t h = L.he_old[he];
+            const int32_t want[5] = {L.iperm[g.he_face[h]], L.iperm[g.ff_plus[2 * size_t(h)]],
+                                     L.iperm[g.ff_plus[2 * size_t(h) + 1]], L.iperm[g.ff_minus[2 * size_t(h)]],
+                                     L.iperm[g.ff_minus[2 * size_t(h) + 1]]};
+            const int32_t got[5] = {row[0] & 0xffff, row[1] & 0xffff, row[1] >> 16, row[2] & 0xffff, row[2] >> 16};
+            for (int q = 0; q < 5; ++q) {
+                require(got[q] >= 0 && got[q] < n_fs, "local face index in range");
+                require(L.fset_face[fs0 + got[q]] == want[q], "local face index names the stencil face");
+            }
+            require((row[0] >> 16) == g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]], "valence");
+        }
         for (int32_t hn = h0; hn < h1; ++hn) {
             const int32_t tw = L.twin_slot[2 * size_t(hn)], slot = L.twin_slot[2 * size_t(hn) + 1];
             require(tw >= 0 && tw < nh && L.twin_slot[2 * size_t(tw)] == hn, "twin is an involution");

@@ -22,7 +22,17 @@ struct Layout {
     std::vector<int32_t> halo_off;     // [n_blocks+1]
     std::vector<int32_t> halo_he;      // internal half-edge ids
     std::vector<int32_t> twin_slot;    // [nh][2]: internal twin, its slot in the owner's block
+    // LDS face sets: block b stages the states of fset_face[fset_off[b] ..
+    // fset_off[b+1]) (its own faces first, then every other face its own and
+    // halo half-edges read); slot rows of block b start at foff[f0] +
+    // halo_off[b] ("extended rows", own then halo, nh + halo_he.size() rows)
+    static constexpr int32_t kFaceSetMax = 512;
+    std::vector<int32_t> fset_off;     // [n_blocks+1]
+    std::vector<int32_t> fset_face;    // internal face ids
+    std::vector<int32_t> ext_loc;      // [n_ext][4]: owner local | valence << 16, plus0 | plus1 << 16,
+                                       //             minus0 | minus1 << 16, internal half-edge id
     int32_t n_blocks() const { return int32_t(blk_face.size()) - 1; }
+    int32_t n_ext() const { return int32_t(ext_loc.size() / 4); }
 };
 
 Layout build_layout(const Geometry& g, int32_t slots);

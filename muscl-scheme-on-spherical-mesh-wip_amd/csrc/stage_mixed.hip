@@ -54,30 +54,28 @@ __device__ __forceinline__ void store_w32(float* __restrict__ W32, int i, const 
     }
 }
 
-// find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433)
+// find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
+// block's LDS face states; `loc` holds the owner / stencil faces' local
+// indices, the float4 rows are the extended-row tables of the half-edge.
 template <int DIM>
-__device__ __forceinline__ void reconstruct32(const GeomPtrs& g, const PhysParams& P, float dt,
-                                              const float* __restrict__ W32, int h, float* out, float4& e4,
-                                              float4& n4) {
-    const int fv = g.he_fv[h];
-    const int i = fv & 0x07ffffff;
-    const int valence = fv >> 27;
-    const int4 st = reinterpret_cast<const int4*>(g.sten)[h];
-    const float4 bt = reinterpret_cast<const float4*>(g.beta32)[h];
-    const float4 hs = reinterpret_cast<const float4*>(g.hsc32)[h];  // 1/H+, 1/H-, BM, arc/area
-    e4 = reinterpret_cast<const float4*>(g.ec32)[h];
-    n4 = reinterpret_cast<const float4*>(g.en32)[h];
-    float wi[DIM], a[DIM], b[DIM], c[DIM], d[DIM];
-    load_w32<DIM>(W32, i, wi);
-    load_w32<DIM>(W32, st.x, a);
-    load_w32<DIM>(W32, st.y, b);
-    load_w32<DIM>(W32, st.z, c);
-    load_w32<DIM>(W32, st.w, d);
+__device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, int4 loc,
+                                                float4 bt, float4 hs, float4 e4, float4 n4, float* out) {
+    constexpr int S4 = DIM == 5 ? 2 : 1;
+    const int valence = loc.x >> 16;
+    const int li[5] = {loc.x & 0xffff, loc.y & 0xffff, loc.y >> 16, loc.z & 0xffff, loc.z >> 16};
+    float w[5][DIM];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const float4 x = sW[li[q] * S4];
+        w[q][0] = x.x; w[q][1] = x.y; w[q][2] = x.z; w[q][3] = x.w;
+        if constexpr (DIM == 5) w[q][4] = sW[li[q] * S4 + 1].x;
+    }
+    const float* wi = w[0];
     float pp[DIM], r[DIM], lim[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-        pp[k] = (bt.x * a[k] + bt.y * b[k] - wi[k]) * hs.x;
-        const float pm = (wi[k] - (bt.z * c[k] + bt.w * d[k])) * hs.y;
+        pp[k] = (bt.x * w[1][k] + bt.y * w[2][k] - wi[k]) * hs.x;
+        const float pm = (wi[k] - (bt.z * w[3][k] + bt.w * w[4][k])) * hs.y;
         r[k] = m_div(pm, pp[k]);
     }
     const V3<float> ec = v3(e4.x, e4.y, e4.z);
@@ -166,15 +164,36 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         area = g.area[i];
     }
 
-    // phase 1: reconstruct own and halo half-edges into LDS
+    // phase 0: stage the FP32 states of the block's face set in LDS (own
+    // faces first); the slot rows are requested first so both loads overlap
+    extern __shared__ float4 sW[];
+    constexpr int S4 = DIM == 5 ? 2 : 1;
     const bool own = t < n_own;
+    const int ext = hbase + hoff + t;
+    int4 loc;
+    float4 bt, hs, e4, n4;
     int2 tw = make_int2(0, 0);
-    float4 e4, n4;
-    if (own) tw = reinterpret_cast<const int2*>(g.twin_slot)[hbase + t];
     if (t < n_slots) {
-        const int h = own ? hbase + t : g.halo_he[hoff + t - n_own];
-        reconstruct32<DIM>(g, P, float(dt), W32, h, &sUP[t * DIM], e4, n4);
+        loc = reinterpret_cast<const int4*>(g.ext_loc)[ext];
+        bt = reinterpret_cast<const float4*>(g.beta32)[ext];
+        hs = reinterpret_cast<const float4*>(g.hsc32)[ext];
+        e4 = reinterpret_cast<const float4*>(g.ec32)[ext];
+        n4 = reinterpret_cast<const float4*>(g.en32)[ext];
     }
+    if (own) tw = reinterpret_cast<const int2*>(g.twin_slot)[hbase + t];
+    {
+        const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
+        const float4* Wv = reinterpret_cast<const float4*>(W32);
+        for (int q = t; q < n_fs; q += kBlock) {
+            const int f = g.fset_face[fs0 + q];
+#pragma unroll
+            for (int c = 0; c < S4; ++c) sW[q * S4 + c] = Wv[size_t(f) * S4 + c];
+        }
+    }
+    __syncthreads();
+
+    // phase 1: reconstruct own and halo half-edges into LDS
+    if (t < n_slots) reconstruct_lds<DIM>(P, float(dt), sW, loc, bt, hs, e4, n4, &sUP[t * DIM]);
     __syncthreads();
 
     // phase 2: one flux per edge, evaluated from the edge's lower-index
@@ -272,12 +291,13 @@ namespace {
 
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s) {
+    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 32 : 16);
     if (stage == 1)
-        k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b, a.W,
-                                                                     a.W2);
+        k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b,
+                                                                       a.W, a.W2);
     else
-        k_stage32<DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W32b, a.W32, a.W2,
-                                                                     a.W);
+        k_stage32<DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32b, a.W32,
+                                                                       a.W2, a.W);
 }
 
 }  // namespace

@@ -126,9 +126,9 @@ class Geometry:
 
 
 def layout_stats(mesh):
-    """(blocks, non-empty blocks, halo half-edges, max halo) of the validated
-    internal GPU layout of `mesh` (host only)."""
-    out = (C.c_int32 * 4)()
+    """(blocks, non-empty blocks, halo half-edges, max halo, max LDS face set)
+    of the validated internal GPU layout of `mesh` (host only)."""
+    out = (C.c_int32 * 5)()
     d = mesh.desc()
     check(lib().splash_layout_stats(C.byref(d), out))
     return tuple(out)

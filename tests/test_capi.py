@@ -81,7 +81,8 @@ def test_gpu_layout_validates_on_every_mesh_family():
     twin slots holding the twins) on every generator."""
     import splash_amd as S
     for mesh in (S.icosphere_hex(5), S.quad_sphere(5), S.uv_sphere(96, 48), S.icosphere(4), S.icosphere_hex(0)):
-        blocks, nonempty, halo, max_halo = S.layout_stats(mesh)
+        blocks, nonempty, halo, max_halo, max_fset = S.layout_stats(mesh)
         assert blocks % 8 == 0 and 0 < nonempty <= blocks
         assert 0 <= halo < mesh.n_halfedges and max_halo < 256
         assert (halo > 0) == (nonempty > 1)
+        assert 0 < max_fset <= 512
