@@ -1,10 +1,17 @@
 """Throughput of the SPLASH MUSCL step on one MI355X.
 
-Workload (default): icosphere_hex(7) (163,842 faces, 983,040 half-edges),
-dim 5, MUSCL_HLLCplus, gam3d = 4/3 (gam = 2 - 1/gam3d), omega_ns = 0,
-accretion off, dt0 = 1e-4; seeded state Sigma = 1 + 0.1 N(0,1), Pi = 0.1
-Sigma, solid-body rotation Omega = 0.5 about z plus 0.01 N(0,1) tangential
-noise.  Both precision paths run the same steps from the same state:
+Default workload (C3, the mixed-precision configuration): quad_sphere(9)
+(1,572,864 faces, 6,291,456 half-edges), dim 5, MUSCL_HLLC, gam3d = 4/3
+(gam = 2 - 1/gam3d), omega_ns = 0.1, accretion on (rate 1e-3, e_acc 0.1,
+omega_acc 0.8, tilt 0.3 rad, width 0.1), friction on with alpha = 1e-6 (the
+shipped parameter file's values), dt0 = 1e-4 (the CFL clamp may lower it:
+the effective dt is reported), 1000 steps; seeded state Sigma = 1 +
+0.1 N(0,1), Pi = 0.1 Sigma, solid-body rotation Omega = 0.5 about z plus
+0.01 N(0,1) tangential noise.  `--workload` selects the other single-mesh
+configurations (c2: icosphere_hex(7) HLLC+; c5a / c5b: uv_sphere(2048,1024)
+/ icosphere_hex(9) HLLC+ with a spherical-harmonic density).
+
+Both precision paths run the same steps from the same state:
   value       = face-updates/s of the FP32-flux / FP64-state path,
   fp64        = face-updates/s of the plain FP64 path,
   l1_err      = relative L1 density error of the FP32-flux path vs FP64.
@@ -27,6 +34,23 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 GAM = 2 - 1 / (4 / 3)
 KERNELS = ["finalize_dt", "stage_1", "stage_2"]
+ACCRETION = dict(accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1, omega_acc_abs=0.8, tilt_angle=0.3,
+                 acc_width=0.1, alpha=1e-6)
+
+# name: (mesh kind, mesh args, solver, omega_ns, accretion, initial state, steps)
+WORKLOADS = {
+    "c3": ("quad", (9,), "HLLC", 0.1, True, "rotation", 1000),
+    "c2": ("hex", (7,), "HLLCplus", 0.0, False, "rotation", 1000),
+    "c5a": ("uv", (2048, 1024), "HLLCplus", 0.1, True, "harmonics", 2000),
+    "c5b": ("hex", (9,), "HLLCplus", 0.1, True, "harmonics", 2000),
+}
+
+# HBM bytes per launch of the FP32-flux stage kernels on the default workload,
+# from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script,
+# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (a 256 MiB device copy reads
+# 128 MiB of FETCH_SIZE and 256 MiB of WRITE_SIZE).  Valid for the kernels of
+# this source tree only; the summary with the commands is under profiles/.
+MEASURED_TRAFFIC = {}
 
 
 def algorithmic_bytes(kernel, F, H, d=5):
@@ -44,9 +68,25 @@ def algorithmic_bytes(kernel, F, H, d=5):
     return 0
 
 
-def run_path(S, mesh, U0, precision, steps, warmup, dt0):
-    s = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, False, params=S.default_params(setup_threads=16),
-                         precision=precision)
+def make_mesh(S, kind, args):
+    return {"hex": S.icosphere_hex, "quad": S.quad_sphere, "uv": S.uv_sphere}[kind](*args)
+
+
+def initial_state(S, mesh, ic):
+    fc = mesh.face_centres()
+    if ic == "rotation":
+        return S.ic.adiabatic_state(fc, GAM, seed=2, sigma_amp=0.1, pi_frac=0.1, omega=(0.0, 0.0, 0.5), noise=0.01)
+    sigma = 1.0 + S.ic.real_sph_harm_sum(fc, seed=5, n_modes=8, lmax=16, amplitude=0.05)
+    return S.ic.adiabatic_state(fc, GAM, seed=5, sigma=sigma, pressure=np.full_like(sigma, 0.1), noise=0.0)
+
+
+def params(S, accretion):
+    return S.default_params(setup_threads=16, **(ACCRETION if accretion else {}))
+
+
+def run_path(S, mesh, U0, wl, precision, steps, warmup, dt0):
+    kind, margs, solver, omega_ns, acc, ic, _ = wl
+    s = getattr(S, "MUSCL_" + solver)(mesh, U0, 5, GAM, omega_ns, acc, params=params(S, acc), precision=precision)
     s.do_steps(dt0, warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -56,10 +96,13 @@ def run_path(S, mesh, U0, precision, steps, warmup, dt0):
     return s, el
 
 
-def cpu_baseline(level, U0, dt0, steps, threads):
+def cpu_baseline(S, wl, U0, dt0, steps, threads):
     import oracle_lib as O
-    m = O.Mesh("hex", level)
-    ref = O.Solver(m, U0, 5, O.HLLCPLUS, GAM, 0.0, threads=threads)
+    kind, margs, solver, omega_ns, acc, ic, _ = wl
+    m = O.Mesh(kind, *margs)
+    p = O.make_params(**{k: v for k, v in ACCRETION.items()}) if acc else O.make_params()
+    osolver = {"HLLCplus": O.HLLCPLUS, "HLLC": O.HLLC}[solver]
+    ref = O.Solver(m, U0, 5, osolver, GAM, omega_ns, p, threads=threads)
     ref.do_step(dt0)  # warm-up, as in the reference timing protocol
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -71,11 +114,12 @@ def cpu_baseline(level, U0, dt0, steps, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's step count)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--level", type=int, default=7)
     ap.add_argument("--dt", type=float, default=1e-4)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-face-updates", type=float, default=3e7,
+                    help="size of the CPU baseline sample in face-updates (whole steps, at least one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -97,71 +141,82 @@ def main():
               file=sys.stderr)
 
     import splash_amd as S
-    mesh = S.icosphere_hex(args.level)
+    wl = WORKLOADS[args.workload]
+    kind, margs, solver, omega_ns, acc, ic, wl_steps = wl
+    steps = args.steps or wl_steps
+    mesh = make_mesh(S, kind, margs)
     F, H = mesh.n_faces, mesh.n_halfedges
-    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=2, sigma_amp=0.1, pi_frac=0.1,
-                              omega=(0.0, 0.0, 0.5), noise=0.01)
+    U0 = initial_state(S, mesh, ic)
 
-    s64, el64 = run_path(S, mesh, U0, S.FP64, args.steps, args.warmup, args.dt)
+    s64, el64 = run_path(S, mesh, U0, wl, S.FP64, steps, args.warmup, args.dt)
     U64 = s64.get_state()
     t64 = s64.time()
     prof64 = s64.profile_step(args.dt, args.profile_steps)
     del s64
-    smx, elmx = run_path(S, mesh, U0, S.MIXED, args.steps, args.warmup, args.dt)
+    smx, elmx = run_path(S, mesh, U0, wl, S.MIXED, steps, args.warmup, args.dt)
     Umx = smx.get_state()
     stop = smx.get_stop_check()
     tmx = smx.time()
+    dt_eff = smx.last_dt()
     prof = smx.profile_step(args.dt, args.profile_steps)
     del smx
 
     l1 = float(np.sum(np.abs(Umx[:, 0] - U64[:, 0])) / np.sum(np.abs(U64[:, 0])))
     linf = float(np.max(np.abs(Umx[:, 0] - U64[:, 0])) / np.max(np.abs(U64[:, 0])))
-    value = F * args.steps / elmx
-    value64 = F * args.steps / el64
+    value = F * steps / elmx
+    value64 = F * steps / el64
 
     k = int(np.argmax(prof[:3]))
     name = KERNELS[k]
     ab = algorithmic_bytes(name, F, H)
     achieved = ab / (prof[k] * 1e-3) / 1e9
     step_bytes = 2 * (104 * H + 36 * F) + 40 * 5 * F
+    traffic = MEASURED_TRAFFIC.get((args.workload, name))
 
+    mesh_name = {"hex": "icosphere_hex", "quad": "quad_sphere", "uv": "uv_sphere"}[kind]
+    state = ("Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.5 z + 0.01N" if ic == "rotation"
+             else "Sigma=1+8 real Y_lm (l<=16, amplitude 0.05), Pi=0.1, at rest")
+    sources = ("accretion on (rate 1e-3, e_acc 0.1, omega_acc 0.8, tilt 0.3, width 0.1), friction alpha 1e-6"
+               if acc else "accretion off")
     line = {
         "metric": "face-updates/s, FP32-flux vs FP64 GPU at <=1e-5 L1 error; %HBM roofline",
         "value": value,
         "unit": "face-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": elmx / args.steps * 1e3,
+        "ms_per_step": elmx / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-flux/f64-state",
         "data": "synthetic (seeded splitmix64/Box-Muller state on the generated mesh)",
         "config": {
-            "workload": f"icosphere_hex({args.level}) F={F} H={H}, dim 5 MUSCL_HLLCplus, gam3d 4/3, "
-                        f"Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.5 z + 0.01N, dt0={args.dt}, accretion off",
-            "faces": F, "halfedges": H, "dt0": args.dt,
+            "workload": f"{args.workload}: {mesh_name}{margs} F={F} H={H}, dim 5 MUSCL_{solver}, gam3d 4/3, "
+                        f"omega_ns={omega_ns}, {state}, {sources}, dt0={args.dt}",
+            "faces": F, "halfedges": H, "dt0": args.dt, "dt_effective": dt_eff,
         },
         "l1_err_vs_fp64": l1,
         "linf_err_vs_fp64": linf,
-        "fp64": {"value": value64, "ms_per_step": el64 / args.steps * 1e3,
+        "speedup_vs_fp64": value / value64,
+        "fp64": {"value": value64, "ms_per_step": el64 / steps * 1e3,
                  "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof64))),
                  "step_hbm_frac": step_bytes / (prof64[3] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
         "step_algorithmic_bytes": step_bytes,
         "step_achieved_GBs": step_bytes / (prof[3] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": ab},
         "stop_check": bool(stop),
         "sim_time": [t64, tmx],
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu_v, cpu_el = cpu_baseline(args.level, U0, args.dt, args.cpu_steps, threads)
+        cpu_steps = max(1, int(round(args.cpu_face_updates / F)))
+        cpu_v, cpu_el = cpu_baseline(S, wl, U0, args.dt, cpu_steps, threads)
         line["cpu_baseline"] = {"value": cpu_v, "unit": "face-updates/s", "cores": threads, "kind": "port",
-                                "sample": f"{args.cpu_steps} FP64 steps of the same workload after 1 warm-up "
+                                "sample": f"{cpu_steps} FP64 steps of the same workload after 1 warm-up "
                                           f"step, oracle (C restatement) with {threads} OpenMP threads, "
                                           f"{cpu_el:.1f} s"}
     print(json.dumps(line))

@@ -202,7 +202,37 @@ void Engine::upload() {
             h32[4 * e + 2] = float(hsc[4 * hn + 2]);
             h32[4 * e + 3] = float(hsc[4 * hn + 3]);
         }
-        d.ext_loc = static_cast<const int32_t*>(dev(sizeof(int32_t) * ext_loc_.size(), ext_loc_.data()));
+        // packed slot records: 9-bit face-set locals, valence, and for own
+        // half-edges the twin's slot and whether this half-edge has the
+        // lower index of its edge (the side that evaluates the flux)
+        std::vector<uint32_t> pk(2 * n_ext, 0u);
+        const int32_t nb = int32_t(blk_face_.size()) - 1;
+        for (int32_t b = 0; b < nb; ++b) {
+            const int32_t h0 = foff_[blk_face_[b]], h1 = foff_[blk_face_[b + 1]];
+            const int32_t n_slots = (h1 - h0) + halo_off_[b + 1] - halo_off_[b];
+            for (int32_t slot = 0; slot < n_slots; ++slot) {
+                const size_t e = size_t(h0) + size_t(halo_off_[b]) + size_t(slot);
+                const int32_t* L = &ext_loc_[4 * e];
+                const uint32_t owner = uint32_t(L[0] & 0xffff), valence = uint32_t(L[0] >> 16);
+                const uint32_t q[4] = {uint32_t(L[1] & 0xffff), uint32_t(L[1] >> 16), uint32_t(L[2] & 0xffff),
+                                       uint32_t(L[2] >> 16)};
+                if (owner > 511 || q[0] > 511 || q[1] > 511 || q[2] > 511 || q[3] > 511 || valence > 31)
+                    throw std::runtime_error("slot record does not fit its 9-bit local indices / 5-bit valence");
+                uint32_t tw = 0, low = 0;
+                if (slot < h1 - h0) {
+                    const int32_t h = h0 + slot;
+                    tw = uint32_t(twin_slot_[2 * size_t(h) + 1]);
+                    low = h < twin_slot_[2 * size_t(h)] ? 1u : 0u;
+                    if (tw > 511) throw std::runtime_error("twin slot does not fit 9 bits");
+                }
+                pk[2 * e] = owner | q[0] << 9 | q[1] << 18 | valence << 27;
+                pk[2 * e + 1] = q[2] | q[3] << 9 | tw << 18 | low << 27;
+            }
+        }
+        d.ext_pk = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * pk.size(), pk.data()));
+        std::vector<double> rate(nf);
+        for (int32_t k = 0; k < nf; ++k) rate[k] = acc[8 * size_t(k)];
+        d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
         d.beta32 = static_cast<const float*>(dev(sizeof(float) * b32.size(), b32.data()));
         d.hsc32 = static_cast<const float*>(dev(sizeof(float) * h32.size(), h32.data()));
         d.ec32 = static_cast<const float*>(dev(sizeof(float) * e32.size(), e32.data()));
@@ -212,11 +242,13 @@ void Engine::upload() {
         int32_t mx = 1;
         for (size_t q = 0; q + 1 < fset_off_.size(); ++q) mx = std::max(mx, fset_off_[q + 1] - fset_off_[q]);
         d.fset_max = mx;
-        const size_t row = dim_ == 5 ? 8 : 4;
+        const size_t row = dim_ == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
-        d.ext_loc = d.fset_off = d.fset_face = nullptr;
+        d.ext_pk = nullptr;
+        d.acc_rate = nullptr;
+        d.fset_off = d.fset_face = nullptr;
         d.beta32 = d.hsc32 = d.ec32 = d.en32 = nullptr;
         d.fset_max = 0;
         args_.W32 = args_.W32b = nullptr;
@@ -237,6 +269,9 @@ void Engine::upload() {
     P.alpha = params_.alpha;
     const double fall_eff = 1;
     P.dmdt_coef = dim_ == 5 ? -(fall_eff * params_.acc_rate * 4 * M_PI) / total_mass_ : 0.0;
+    P.om_acc[0] = 0;
+    P.om_acc[1] = std::sin(params_.tilt_angle) * params_.omega_acc_abs;
+    P.om_acc[2] = std::cos(params_.tilt_angle) * params_.omega_acc_abs;
 
     args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
     args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));

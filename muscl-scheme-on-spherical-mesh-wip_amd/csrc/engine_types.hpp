@@ -36,6 +36,7 @@ struct PhysParams {
     double h0, round_diff;
     double e_acc, alpha;
     double dmdt_coef;   // -(fall_eff*acc_rate*4*pi)/total_mass (adiabatic.hpp:463)
+    double om_acc[3];   // accretion angular velocity (0, sin(tilt), cos(tilt)) * omega_acc_abs
 };
 
 // Device tables in the internal face order (faces sorted along a Morton
@@ -61,13 +62,18 @@ struct GeomPtrs {
     // mixed-precision kernel only (null in FP64 runs): FP32 rows per
     // extended slot row (block b's own then halo half-edges start at
     // foff[f0] + halo_off[b]) and the LDS face sets
-    const int32_t* ext_loc;   // [n_ext][4]: owner local | valence << 16, plus0 | plus1 << 16, minus0 | minus1 << 16, he
+    // ext_pk [n_ext][2] packs 9-bit face-set local indices:
+    //   x = owner | plus0 << 9 | plus1 << 18 | valence << 27
+    //   y = minus0 | minus1 << 9 | twin slot << 18 | lower-index-of-edge << 27
+    // (twin fields only for own half-edges)
+    const uint32_t* ext_pk;
     const float* beta32;      // [n_ext][4]
     const float* hsc32;       // [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
     const float* ec32;        // [n_ext][4]: unit edge midpoint xyz, |ec|
     const float* en32;        // [n_ext][4]: edge normal xyz, chord length
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
+    const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt

@@ -15,7 +15,7 @@ struct StepArgs {
     double* U;  // [nf][dim] E-form state
     double* W;   // [nf][dim] p-form input of stage 1 (and output of stage 2)
     double* W2;  // [nf][dim] p-form input of stage 2
-    float* W32;  // mixed runs: [nf][8] (dim 5) or [nf][4] (dim 4) FP32 copy of the stage input
+    float* W32;  // mixed runs: FP32 copy of the stage input, [nf] float4 (Sigma, L) then (dim 5) [nf] pressure
     float* W32b; // mixed runs: the other stage's FP32 input
 };
 

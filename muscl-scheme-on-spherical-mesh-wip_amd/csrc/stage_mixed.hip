@@ -26,49 +26,36 @@ __device__ __forceinline__ float m_tanh(float x) {
 
 // ----------------------------------------------------------------------
 // The mixed-precision stage kernel.  It follows k_stage (same patches, LDS
-// slots and face loop) but reads FP32 copies: the stage input as 8 floats
-// per face (two float4 loads per stencil gather) and one float4 record per
-// half-edge table, so a half-edge costs 5 vector loads instead of 10 and
-// 25 scalar gathers.  Face areas, the state, the source terms, the dt
-// reductions and the flux accumulation (compensated, Neumaier) stay FP64.
+// slots and face loop) but works on FP32 copies: the stage input as one
+// float4 (Sigma, L) row per face plus (dim 5) one pressure per face, staged
+// per block in LDS, and four float4 rows plus one packed 8-byte record per
+// slot.  Face areas, the state, the source terms, the dt reductions and the
+// flux accumulation (compensated, Neumaier) stay FP64.
+
+static_assert(kBlock <= 512, "slot indices are packed in 9 bits");
 
 template <int DIM>
-__device__ __forceinline__ void load_w32(const float* __restrict__ W32, int i, float* w) {
-    if constexpr (DIM == 5) {
-        const float4 a = reinterpret_cast<const float4*>(W32)[2 * size_t(i)];
-        const float4 b = reinterpret_cast<const float4*>(W32)[2 * size_t(i) + 1];
-        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x;
-    } else {
-        const float4 a = reinterpret_cast<const float4*>(W32)[size_t(i)];
-        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
-    }
-}
-
-template <int DIM>
-__device__ __forceinline__ void store_w32(float* __restrict__ W32, int i, const double* w) {
-    if constexpr (DIM == 5) {
-        reinterpret_cast<float4*>(W32)[2 * size_t(i)] = make_float4(float(w[0]), float(w[1]), float(w[2]), float(w[3]));
-        reinterpret_cast<float4*>(W32)[2 * size_t(i) + 1] = make_float4(float(w[4]), 0.f, 0.f, 0.f);
-    } else {
-        reinterpret_cast<float4*>(W32)[size_t(i)] = make_float4(float(w[0]), float(w[1]), float(w[2]), float(w[3]));
-    }
+__device__ __forceinline__ void store_w32(float* __restrict__ W32, int nf, int i, const double* w) {
+    reinterpret_cast<float4*>(W32)[i] = make_float4(float(w[0]), float(w[1]), float(w[2]), float(w[3]));
+    if constexpr (DIM == 5) W32[4 * size_t(nf) + i] = float(w[4]);
 }
 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
-// block's LDS face states; `loc` holds the owner / stencil faces' local
-// indices, the float4 rows are the extended-row tables of the half-edge.
+// block's LDS face states (sW: Sigma, L; sP: pressure); `pk` is the slot's
+// packed record (owner and stencil faces' local indices, valence), the
+// float4 rows are its beta / {1/H+, 1/H-, BM, arc/area} / ec / en rows.
 template <int DIM>
-__device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, int4 loc,
-                                                float4 bt, float4 hs, float4 e4, float4 n4, float* out) {
-    constexpr int S4 = DIM == 5 ? 2 : 1;
-    const int valence = loc.x >> 16;
-    const int li[5] = {loc.x & 0xffff, loc.y & 0xffff, loc.y >> 16, loc.z & 0xffff, loc.z >> 16};
+__device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, const float* sP,
+                                                uint2 pk, float4 bt, float4 hs, float4 e4, float4 n4, float* out) {
+    const int valence = int(pk.x >> 27);
+    const int li[5] = {int(pk.x & 511u), int((pk.x >> 9) & 511u), int((pk.x >> 18) & 511u), int(pk.y & 511u),
+                       int((pk.y >> 9) & 511u)};
     float w[5][DIM];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        const float4 x = sW[li[q] * S4];
+        const float4 x = sW[li[q]];
         w[q][0] = x.x; w[q][1] = x.y; w[q][2] = x.z; w[q][3] = x.w;
-        if constexpr (DIM == 5) w[q][4] = sW[li[q] * S4 + 1].x;
+        if constexpr (DIM == 5) w[q][4] = sP[li[q]];
     }
     const float* wi = w[0];
     float pp[DIM], r[DIM], lim[DIM];
@@ -97,6 +84,69 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         // E_fc (MUSCL_base.hpp:541-562)
         const V3<float> vel = cross(ec, v3(out[1], out[2], out[3])) / (-out[0]);
         out[4] = out[4] * float(1 / (P.gam - 1)) + out[0] * dot(vel, vel) * 0.5f;
+    }
+}
+
+// The source terms of adiabatic.hpp:396-541 (as source_adiabatic) in FP64
+// on the stage input's p-form state w, with the accretion vectors
+// omega_acc x r and r x (omega_acc x r) formed from the face centre, one
+// reciprocal per density / pressure instead of a division per use, and
+// x^(1/4) = sqrt(sqrt(x)), y^4 = (y^2)^2 in place of pow.
+__device__ __forceinline__ void source_mixed(const PhysParams& P, const double* w, V3<double> fcn, double rate,
+                                             double* res) {
+    const double gam = P.gam;
+    const double inv0 = 1.0 / w[0], invp = 1.0 / w[4];
+    const V3<double> L = v3(w[1], w[2], w[3]);
+    const V3<double> vel = cross(fcn, L) * (-inv0);
+    const V3<double> omxr = cross(v3(P.om_acc[0], P.om_acc[1], P.om_acc[2]), fcn);
+    const V3<double> rxo = cross(fcn, omxr);
+    res[0] = rate;
+    res[1] = rate * rxo.x;
+    res[2] = rate * rxo.y;
+    res[3] = rate * rxo.z;
+    const V3<double> dvv = omxr - vel;
+    res[4] = rate * (P.e_acc + dot(dvv, dvv) * 0.5);
+    const double vn2 = dot(vel, vel);
+    const double h_spec = gam / (gam - 1) * w[4] * inv0;  // gam/(gam-1) p/Sigma
+    if (!P.friction_on) {
+        const V3<double> rxv = cross(fcn, vel);
+        const double dmdt = P.dmdt_coef * w[0];
+        res[0] += dmdt;
+        res[1] += dmdt * rxv.x;
+        res[2] += dmdt * rxv.y;
+        res[3] += dmdt * rxv.z;
+        res[4] += dmdt * (h_spec + vn2 * 0.5);
+    }
+    const double g_eff = kGM - vn2;
+    const double c_sigma = 4.85e36, k_m = 1.6e-13, kappa = 3.4e6;
+    const double C = (12. / 5 * k_m) * w[0] * invp * sqrt(sqrt((3. / 4 * c_sigma) * g_eff * w[0]));
+    const double beta_switch = 0.736194670678821;
+    const double C_switch = -1 - 1 / (beta_switch - 1);
+    double beta;
+    if (C <= C_switch) {
+        beta = 1 - 1 / (1 + C);
+    } else {
+        const double q = 2 / C, q2 = q * q;
+        beta = 1 - q2 * q2;
+    }
+    const double beta_ceil = 1 - 1e-8, beta_floor = 1e-8;
+    if (beta < beta_floor || __builtin_isnan(beta)) beta = beta_floor;
+    if (beta > beta_ceil) beta = beta_ceil;
+    res[4] -= g_eff * (1 / kappa) * (1 - beta);
+    if (P.friction_on) {
+        const double gam3d = 1 / (2 - gam);
+        const double rho = gam3d / (2 * gam3d - 1) * g_eff * w[0] * w[0] * invp;
+        const V3<double> vel0 = cross(v3(0.0, 0.0, P.omega_ns), fcn);
+        const V3<double> dv0 = vel - vel0;
+        const double a = P.alpha * rho * sqrt(dot(dv0, dv0));
+        const V3<double> wr = dv0 * (-a);
+        const V3<double> l_fr = cross(fcn, wr);
+        const double a0 = a * inv0;
+        res[0] -= a;
+        res[1] += l_fr.x - a0 * w[1];
+        res[2] += l_fr.y - a0 * w[2];
+        res[3] += l_fr.z - a0 * w[3];
+        res[4] += dot(wr, vel0) - a * (vn2 * 0.5 + h_spec);
     }
 }
 
@@ -135,8 +185,7 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
 template <int DIM, int SOLVER, int STAGE>
 __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Scalars* __restrict__ sc,
                                                     double* __restrict__ U, const float* __restrict__ W32,
-                                                    float* __restrict__ W32_out, const double* __restrict__ W,
-                                                    double* __restrict__ W_out) {
+                                                    float* __restrict__ W32_out) {
     __shared__ float sUP[kBlock * DIM];
     __shared__ float sF[kBlock * DIM];  // edge flux times chord length, per own slot
     const int b = xcd_block();
@@ -151,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     // face data of phase 3, requested before the stencil gathers
     const int i = f0 + t;
     const bool face = i < f1;
-    double u0[DIM], area = 1;
+    double u0[DIM], area = 1, rate = 0;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
     if (face) {
@@ -162,38 +211,39 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         const double4 c4 = reinterpret_cast<const double4*>(g.fcn)[i];
         fcn = v3(c4.x, c4.y, c4.z);
         area = g.area[i];
+        if (DIM == 5 && P.accretion_on) rate = g.acc_rate[i];
     }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
-    // faces first); the slot rows are requested first so both loads overlap
+    // faces first, so face f0 + t sits at local index t); the slot rows are
+    // requested first so both loads overlap
     extern __shared__ float4 sW[];
-    constexpr int S4 = DIM == 5 ? 2 : 1;
+    float* sP = reinterpret_cast<float*>(sW + g.fset_max);
     const bool own = t < n_own;
     const int ext = hbase + hoff + t;
-    int4 loc;
+    uint2 pk = make_uint2(0u, 0u);
     float4 bt, hs, e4, n4;
-    int2 tw = make_int2(0, 0);
     if (t < n_slots) {
-        loc = reinterpret_cast<const int4*>(g.ext_loc)[ext];
+        pk = reinterpret_cast<const uint2*>(g.ext_pk)[ext];
         bt = reinterpret_cast<const float4*>(g.beta32)[ext];
         hs = reinterpret_cast<const float4*>(g.hsc32)[ext];
         e4 = reinterpret_cast<const float4*>(g.ec32)[ext];
         n4 = reinterpret_cast<const float4*>(g.en32)[ext];
     }
-    if (own) tw = reinterpret_cast<const int2*>(g.twin_slot)[hbase + t];
     {
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
-        const float4* Wv = reinterpret_cast<const float4*>(W32);
+        const float4* W4 = reinterpret_cast<const float4*>(W32);
+        const float* Wp = W32 + 4 * size_t(g.nf);
         for (int q = t; q < n_fs; q += kBlock) {
             const int f = g.fset_face[fs0 + q];
-#pragma unroll
-            for (int c = 0; c < S4; ++c) sW[q * S4 + c] = Wv[size_t(f) * S4 + c];
+            sW[q] = W4[f];
+            if constexpr (DIM == 5) sP[q] = Wp[f];
         }
     }
     __syncthreads();
 
     // phase 1: reconstruct own and halo half-edges into LDS
-    if (t < n_slots) reconstruct_lds<DIM>(P, float(dt), sW, loc, bt, hs, e4, n4, &sUP[t * DIM]);
+    if (t < n_slots) reconstruct_lds<DIM>(P, float(dt), sW, sP, pk, bt, hs, e4, n4, &sUP[t * DIM]);
     __syncthreads();
 
     // phase 2: one flux per edge, evaluated from the edge's lower-index
@@ -202,14 +252,15 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     // exchange exactly opposite fluxes
     if (own) {
         const int h = hbase + t;
-        const bool low = h < tw.x;
-        if (low || tw.y >= n_own) {
+        const int tws = int((pk.y >> 18) & 511u);
+        const bool low = (pk.y >> 27) & 1u;
+        if (low || tws >= n_own) {
             Edge<float> e;
             e.ec = v3(e4.x, e4.y, e4.z);
             e.ecn = e4.w;
             e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
             float uL[DIM], uR[DIM], F[DIM];
-            const int sl = low ? t : tw.y, sr = low ? tw.y : t;
+            const int sl = low ? t : tws, sr = low ? tws : t;
 #pragma unroll
             for (int k = 0; k < DIM; ++k) {
                 uL[k] = sUP[sl * DIM + k];
@@ -220,9 +271,9 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
             const float chord = low ? n4.w : -n4.w;
 #pragma unroll
             for (int k = 0; k < DIM; ++k) sF[t * DIM + k] = F[k] * chord;
-            if (low && tw.y < n_own)
+            if (low && tws < n_own)
 #pragma unroll
-                for (int k = 0; k < DIM; ++k) sF[tw.y * DIM + k] = -(F[k] * chord);
+                for (int k = 0; k < DIM; ++k) sF[tws * DIM + k] = -(F[k] * chord);
         }
     }
     __syncthreads();
@@ -237,10 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         for (int k = 0; k < DIM; ++k) res[k] = 0;
         if constexpr (DIM == 5) {
             if (P.accretion_on) {
-                double w[5], src[5];
-#pragma unroll
-                for (int k = 0; k < 5; ++k) w[k] = W[size_t(i) * DIM + k];
-                source_adiabatic(P, w, fcn, g.acc + 8 * size_t(i), src);
+                // the stage input (p-form) of this face, from LDS
+                const float4 x = sW[t];
+                const double w[5] = {x.x, x.y, x.z, x.w, sP[t]};
+                double src[5];
+                source_mixed(P, w, fcn, rate, src);
 #pragma unroll
                 for (int k = 0; k < 5; ++k) res[k] = dt_here * src[k];
             }
@@ -268,11 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         if (un[0] < 1e-8) un[0] = 1e-8;
         double wn[DIM];
         face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
-        store_w32<DIM>(W32_out, i, wn);
-        if (P.accretion_on) {
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) W_out[size_t(i) * DIM + k] = wn[k];
-        }
+        store_w32<DIM>(W32_out, g.nf, i, wn);
         if constexpr (STAGE == 2) {
 #pragma unroll
             for (int k = 0; k < DIM; ++k) U[size_t(i) * DIM + k] = un[k];
@@ -291,13 +339,11 @@ namespace {
 
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s) {
-    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 32 : 16);
+    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 20 : 16);
     if (stage == 1)
-        k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b,
-                                                                       a.W, a.W2);
+        k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b);
     else
-        k_stage32<DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32b, a.W32,
-                                                                       a.W2, a.W);
+        k_stage32<DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32b, a.W32);
 }
 
 }  // namespace
