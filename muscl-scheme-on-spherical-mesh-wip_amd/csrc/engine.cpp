@@ -186,57 +186,86 @@ void Engine::upload() {
     d.halo_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_off_.size(), halo_off_.data()));
     d.halo_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_he_.size(), halo_he_.data()));
     if (precision_ == kMixed) {
-        // FP32 rows of the mixed kernel per extended slot row: beta,
-        // {1/H+, 1/H-, BM, arc/area}, {ec, |ec|}, {en, chord}
+        // FP32 tables of the mixed kernel, per extended slot row (block b's
+        // own then halo half-edges start at foff[f0] + halo_off[b]):
+        //   slot_a: packed record (x, y) and beta+0, beta-0 (beta.1 = 1 - beta.0
+        //           to 3e-13 on every generated mesh, invisible in FP32)
+        //   slot_b: 1/H+, 1/H-, BM_dist, arc/area
+        // and per edge of a block (a block holds both slots of each of its
+        // edges, so it has n_slots/2 edges, starting at row (foff[f0] +
+        // halo_off[b]) / 2), oriented for the slot that evaluates its flux:
+        //   edge:   {ec xyz, |ec|}, {en xyz, chord}
+        // Record bits: x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27;
+        // y = minus1 | (own: twin slot, halo: owner local) << 9 | low << 18 |
+        // edge << 19 | negate-en << 26.  An own slot finds its owner in LDS.
         const size_t n_ext = ext_loc_.size() / 4;
-        std::vector<float> b32(4 * n_ext), h32(4 * n_ext), e32(4 * n_ext), n32(4 * n_ext);
-        for (size_t e = 0; e < n_ext; ++e) {
-            const size_t hn = size_t(ext_loc_[4 * e + 3]);
-            for (int c = 0; c < 4; ++c) {
-                b32[4 * e + c] = float(beta[4 * hn + c]);
-                e32[4 * e + c] = float(ec[4 * hn + c]);
-                n32[4 * e + c] = float(en[4 * hn + c]);
-            }
-            h32[4 * e + 0] = float(1.0 / hsc[4 * hn + 0]);
-            h32[4 * e + 1] = float(1.0 / hsc[4 * hn + 1]);
-            h32[4 * e + 2] = float(hsc[4 * hn + 2]);
-            h32[4 * e + 3] = float(hsc[4 * hn + 3]);
-        }
-        // packed slot records: 9-bit face-set locals, valence, and for own
-        // half-edges the twin's slot and whether this half-edge has the
-        // lower index of its edge (the side that evaluates the flux)
-        std::vector<uint32_t> pk(2 * n_ext, 0u);
+        if (n_ext % 2) throw std::runtime_error("odd slot count");
+        std::vector<uint32_t> sa(4 * n_ext, 0u);
+        std::vector<float> sb(4 * n_ext), ed(4 * n_ext);  // ed: n_ext/2 edges of 8 floats
         const int32_t nb = int32_t(blk_face_.size()) - 1;
+        std::vector<int32_t> eidx, own_of_halo;
         for (int32_t b = 0; b < nb; ++b) {
             const int32_t h0 = foff_[blk_face_[b]], h1 = foff_[blk_face_[b + 1]];
-            const int32_t n_slots = (h1 - h0) + halo_off_[b + 1] - halo_off_[b];
-            for (int32_t slot = 0; slot < n_slots; ++slot) {
-                const size_t e = size_t(h0) + size_t(halo_off_[b]) + size_t(slot);
+            const int32_t n_own = h1 - h0;
+            const int32_t n_slots = n_own + halo_off_[b + 1] - halo_off_[b];
+            const size_t e0 = size_t(h0) + size_t(halo_off_[b]);
+            if (n_slots % 2 || e0 % 2) throw std::runtime_error("block slots do not pair into edges");
+            eidx.assign(size_t(n_slots), -1);
+            own_of_halo.assign(size_t(n_slots), -1);
+            auto tw_of = [&](int32_t s) { return twin_slot_[2 * size_t(h0 + s) + 1]; };
+            auto low_of = [&](int32_t s) { return h0 + s < twin_slot_[2 * size_t(h0 + s)]; };
+            int32_t n_edges = 0;
+            for (int32_t s = 0; s < n_own; ++s)  // representatives: the slots that evaluate a flux
+                if (low_of(s) || tw_of(s) >= n_own) eidx[size_t(s)] = n_edges++;
+            if (2 * n_edges != n_slots) throw std::runtime_error("edge count is not half the slot count");
+            for (int32_t s = 0; s < n_own; ++s) {
+                const int32_t t = tw_of(s);
+                if (t >= n_own) own_of_halo[size_t(t)] = s;
+                else if (eidx[size_t(s)] < 0) eidx[size_t(s)] = eidx[size_t(t)];
+            }
+            for (int32_t s = n_own; s < n_slots; ++s) {
+                if (own_of_halo[size_t(s)] < 0) throw std::runtime_error("halo slot without an own twin");
+                eidx[size_t(s)] = eidx[size_t(own_of_halo[size_t(s)])];
+            }
+            for (int32_t s = 0; s < n_slots; ++s) {
+                const size_t e = e0 + size_t(s);
                 const int32_t* L = &ext_loc_[4 * e];
+                const size_t hn = size_t(L[3]);
                 const uint32_t owner = uint32_t(L[0] & 0xffff), valence = uint32_t(L[0] >> 16);
                 const uint32_t q[4] = {uint32_t(L[1] & 0xffff), uint32_t(L[1] >> 16), uint32_t(L[2] & 0xffff),
                                        uint32_t(L[2] >> 16)};
                 if (owner > 511 || q[0] > 511 || q[1] > 511 || q[2] > 511 || q[3] > 511 || valence > 31)
                     throw std::runtime_error("slot record does not fit its 9-bit local indices / 5-bit valence");
-                uint32_t tw = 0, low = 0;
-                if (slot < h1 - h0) {
-                    const int32_t h = h0 + slot;
-                    tw = uint32_t(twin_slot_[2 * size_t(h) + 1]);
-                    low = h < twin_slot_[2 * size_t(h)] ? 1u : 0u;
-                    if (tw > 511) throw std::runtime_error("twin slot does not fit 9 bits");
+                const bool own = s < n_own;
+                const bool rep = own && (low_of(s) || tw_of(s) >= n_own);
+                const uint32_t mid = own ? uint32_t(tw_of(s)) : owner;
+                const uint32_t low = own && low_of(s) ? 1u : 0u;
+                const uint32_t ei = uint32_t(eidx[size_t(s)]);
+                if (mid > 511 || ei > 127) throw std::runtime_error("slot record does not fit its fields");
+                sa[4 * e] = q[0] | q[1] << 9 | q[2] << 18 | valence << 27;
+                sa[4 * e + 1] = q[3] | mid << 9 | low << 18 | ei << 19 | (rep ? 0u : 1u) << 26;
+                const float bp = float(beta[4 * hn + 0]), bm = float(beta[4 * hn + 2]);
+                std::memcpy(&sa[4 * e + 2], &bp, 4);
+                std::memcpy(&sa[4 * e + 3], &bm, 4);
+                sb[4 * e + 0] = float(1.0 / hsc[4 * hn + 0]);
+                sb[4 * e + 1] = float(1.0 / hsc[4 * hn + 1]);
+                sb[4 * e + 2] = float(hsc[4 * hn + 2]);
+                sb[4 * e + 3] = float(hsc[4 * hn + 3]);
+                if (rep) {
+                    float* r = &ed[8 * (e0 / 2 + ei)];
+                    for (int c = 0; c < 4; ++c) {
+                        r[c] = float(ec[4 * hn + c]);
+                        r[4 + c] = float(en[4 * hn + c]);
+                    }
                 }
-                pk[2 * e] = owner | q[0] << 9 | q[1] << 18 | valence << 27;
-                pk[2 * e + 1] = q[2] | q[3] << 9 | tw << 18 | low << 27;
             }
         }
-        d.ext_pk = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * pk.size(), pk.data()));
         std::vector<double> rate(nf);
         for (int32_t k = 0; k < nf; ++k) rate[k] = acc[8 * size_t(k)];
         d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
-        d.beta32 = static_cast<const float*>(dev(sizeof(float) * b32.size(), b32.data()));
-        d.hsc32 = static_cast<const float*>(dev(sizeof(float) * h32.size(), h32.data()));
-        d.ec32 = static_cast<const float*>(dev(sizeof(float) * e32.size(), e32.data()));
-        d.en32 = static_cast<const float*>(dev(sizeof(float) * n32.size(), n32.data()));
+        d.slot_a = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * sa.size(), sa.data()));
+        d.slot_b = static_cast<const float*>(dev(sizeof(float) * sb.size(), sb.data()));
+        d.edge32 = static_cast<const float*>(dev(sizeof(float) * ed.size(), ed.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off_.size(), fset_off_.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face_.size(), fset_face_.data()));
         int32_t mx = 1;
@@ -246,10 +275,10 @@ void Engine::upload() {
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
-        d.ext_pk = nullptr;
+        d.slot_a = nullptr;
         d.acc_rate = nullptr;
         d.fset_off = d.fset_face = nullptr;
-        d.beta32 = d.hsc32 = d.ec32 = d.en32 = nullptr;
+        d.slot_b = d.edge32 = nullptr;
         d.fset_max = 0;
         args_.W32 = args_.W32b = nullptr;
     }

@@ -59,18 +59,20 @@ struct GeomPtrs {
     const int32_t* blk_face;  // [n_blocks+1] face range of each stage block
     const int32_t* halo_off;  // [n_blocks+1] range of each block's halo in halo_he
     const int32_t* halo_he;   // outside twins of each block's boundary half-edges
-    // mixed-precision kernel only (null in FP64 runs): FP32 rows per
-    // extended slot row (block b's own then halo half-edges start at
-    // foff[f0] + halo_off[b]) and the LDS face sets
-    // ext_pk [n_ext][2] packs 9-bit face-set local indices:
-    //   x = owner | plus0 << 9 | plus1 << 18 | valence << 27
-    //   y = minus0 | minus1 << 9 | twin slot << 18 | lower-index-of-edge << 27
-    // (twin fields only for own half-edges)
-    const uint32_t* ext_pk;
-    const float* beta32;      // [n_ext][4]
-    const float* hsc32;       // [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
-    const float* ec32;        // [n_ext][4]: unit edge midpoint xyz, |ec|
-    const float* en32;        // [n_ext][4]: edge normal xyz, chord length
+    // mixed-precision kernel only (null in FP64 runs).  Per extended slot
+    // row (block b's own then halo half-edges start at foff[f0] +
+    // halo_off[b]):
+    //   slot_a [n_ext][4]: packed record x, y; beta+0, beta-0 (float bits)
+    //     x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27
+    //     y = minus1 | (own: twin slot | halo: owner local) << 9 | low << 18 |
+    //         edge << 19 | negate-en << 26   (stencil fields: face-set locals)
+    //   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
+    // and per edge of a block (n_slots/2 edges from row (foff[f0] +
+    // halo_off[b]) / 2, oriented for the slot that evaluates the flux):
+    //   edge32 [n_ext/2][8]: unit edge midpoint xyz, |ec|, normal xyz, chord
+    const uint32_t* slot_a;
+    const float* slot_b;
+    const float* edge32;
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])

@@ -41,15 +41,14 @@ __device__ __forceinline__ void store_w32(float* __restrict__ W32, int nf, int i
 }
 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
-// block's LDS face states (sW: Sigma, L; sP: pressure); `pk` is the slot's
-// packed record (owner and stencil faces' local indices, valence), the
-// float4 rows are its beta / {1/H+, 1/H-, BM, arc/area} / ec / en rows.
+// block's LDS face states (sW: Sigma, L; sP: pressure): li = owner, plus0,
+// plus1, minus0, minus1 local indices; bp, bm = beta+0, beta-0 (the pair's
+// second weight is 1 - the first); hs = {1/H+, 1/H-, BM, arc/area}; e4 =
+// {ec, |ec|}; en = the half-edge's outward normal.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, const float* sP,
-                                                uint2 pk, float4 bt, float4 hs, float4 e4, float4 n4, float* out) {
-    const int valence = int(pk.x >> 27);
-    const int li[5] = {int(pk.x & 511u), int((pk.x >> 9) & 511u), int((pk.x >> 18) & 511u), int(pk.y & 511u),
-                       int((pk.y >> 9) & 511u)};
+                                                const int* li, int valence, float bp, float bm, float4 hs, float4 e4,
+                                                V3<float> en, float* out) {
     float w[5][DIM];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -61,12 +60,11 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
     float pp[DIM], r[DIM], lim[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-        pp[k] = (bt.x * w[1][k] + bt.y * w[2][k] - wi[k]) * hs.x;
-        const float pm = (wi[k] - (bt.z * w[3][k] + bt.w * w[4][k])) * hs.y;
+        pp[k] = ((w[2][k] + bp * (w[1][k] - w[2][k])) - wi[k]) * hs.x;
+        const float pm = (wi[k] - (w[4][k] + bm * (w[3][k] - w[4][k]))) * hs.y;
         r[k] = m_div(pm, pp[k]);
     }
     const V3<float> ec = v3(e4.x, e4.y, e4.z);
-    const V3<float> en = v3(n4.x, n4.y, n4.z);
     const float eta_m = __builtin_amdgcn_rcpf(hs.z * hs.y), eta_p = __builtin_amdgcn_rcpf(hs.z * hs.x);  // H-+/BM
     if constexpr (DIM == 5) {
         const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0] * e4.w);
@@ -215,20 +213,21 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
-    // faces first, so face f0 + t sits at local index t); the slot rows are
-    // requested first so both loads overlap
+    // faces first, so face f0 + t sits at local index t) and each own slot's
+    // owner; the slot and edge rows are requested first so all loads overlap
     extern __shared__ float4 sW[];
     float* sP = reinterpret_cast<float*>(sW + g.fset_max);
+    __shared__ uint8_t sOwn[kBlock];
     const bool own = t < n_own;
     const int ext = hbase + hoff + t;
-    uint2 pk = make_uint2(0u, 0u);
-    float4 bt, hs, e4, n4;
+    uint4 ra = make_uint4(0u, 0u, 0u, 0u);
+    float4 hs, e4, n4;
     if (t < n_slots) {
-        pk = reinterpret_cast<const uint2*>(g.ext_pk)[ext];
-        bt = reinterpret_cast<const float4*>(g.beta32)[ext];
-        hs = reinterpret_cast<const float4*>(g.hsc32)[ext];
-        e4 = reinterpret_cast<const float4*>(g.ec32)[ext];
-        n4 = reinterpret_cast<const float4*>(g.en32)[ext];
+        ra = reinterpret_cast<const uint4*>(g.slot_a)[ext];
+        hs = reinterpret_cast<const float4*>(g.slot_b)[ext];
+        const int edge = ((hbase + hoff) >> 1) + int((ra.y >> 19) & 127u);
+        e4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge)];
+        n4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge) + 1];
     }
     {
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
@@ -240,41 +239,60 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
             if constexpr (DIM == 5) sP[q] = Wp[f];
         }
     }
+    if (face)
+        for (int hh = hb; hh < he; ++hh) sOwn[hh - hbase] = uint8_t(t);
+    const int mid = int((ra.y >> 9) & 511u);  // own: twin slot; halo: owner local
     __syncthreads();
 
-    // phase 1: reconstruct own and halo half-edges into LDS
-    if (t < n_slots) reconstruct_lds<DIM>(P, float(dt), sW, sP, pk, bt, hs, e4, n4, &sUP[t * DIM]);
+    // phase 1: reconstruct own and halo half-edges into LDS; the slot that
+    // evaluates an edge's flux records itself under the edge's index, and
+    // thread k requests the rows of edge k for phase 2
+    __shared__ uint32_t sRep[kBlock / 2];
+    const int n_edges = n_slots >> 1;
+    if (t < n_slots) {
+        const int li[5] = {own ? int(sOwn[t]) : mid, int(ra.x & 511u), int((ra.x >> 9) & 511u),
+                           int((ra.x >> 18) & 511u), int(ra.y & 511u)};
+        const bool rep = !((ra.y >> 26) & 1u);
+        const float sg = rep ? 1.0f : -1.0f;
+        reconstruct_lds<DIM>(P, float(dt), sW, sP, li, int(ra.x >> 27), __uint_as_float(ra.z),
+                             __uint_as_float(ra.w), hs, e4, v3(sg * n4.x, sg * n4.y, sg * n4.z), &sUP[t * DIM]);
+        if (rep) sRep[(ra.y >> 19) & 127u] = uint32_t(t) | (ra.y & (1023u << 9));  // slot | twin slot, low
+    }
+    if (t < n_edges) {
+        const int edge = ((hbase + hoff) >> 1) + t;
+        e4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge)];
+        n4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge) + 1];
+    }
     __syncthreads();
 
-    // phase 2: one flux per edge, evaluated from the edge's lower-index
-    // half-edge (the twin side has the same midpoint and the negated normal)
-    // and written as +F*chord / -F*chord, so the two faces of an edge
-    // exchange exactly opposite fluxes
-    if (own) {
-        const int h = hbase + t;
-        const int tws = int((pk.y >> 18) & 511u);
-        const bool low = (pk.y >> 27) & 1u;
-        if (low || tws >= n_own) {
-            Edge<float> e;
-            e.ec = v3(e4.x, e4.y, e4.z);
-            e.ecn = e4.w;
-            e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
-            float uL[DIM], uR[DIM], F[DIM];
-            const int sl = low ? t : tws, sr = low ? tws : t;
+    // phase 2: one flux per edge, thread k for edge k, evaluated from the
+    // edge's lower-index half-edge (the twin side has the same midpoint and
+    // the negated normal) and written as +F*chord / -F*chord, so the two
+    // faces of an edge exchange exactly opposite fluxes
+    if (t < n_edges) {
+        const uint32_t rec = sRep[t];
+        const int sw = int(rec & 511u);          // the evaluating (own) slot
+        const int tws = int((rec >> 9) & 511u);  // its twin's slot
+        const bool low = (rec >> 18) & 1u;
+        Edge<float> e;
+        e.ec = v3(e4.x, e4.y, e4.z);
+        e.ecn = e4.w;
+        e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
+        float uL[DIM], uR[DIM], F[DIM];
+        const int sl = low ? sw : tws, sr = low ? tws : sw;
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) {
-                uL[k] = sUP[sl * DIM + k];
-                uR[k] = sUP[sr * DIM + k];
-            }
-            const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
-            if (err) raise_stop(sc, err, h);
-            const float chord = low ? n4.w : -n4.w;
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) sF[t * DIM + k] = F[k] * chord;
-            if (low && tws < n_own)
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) sF[tws * DIM + k] = -(F[k] * chord);
+        for (int k = 0; k < DIM; ++k) {
+            uL[k] = sUP[sl * DIM + k];
+            uR[k] = sUP[sr * DIM + k];
         }
+        const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
+        if (err) raise_stop(sc, err, hbase + sw);
+        const float chord = low ? n4.w : -n4.w;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) sF[sw * DIM + k] = F[k] * chord;
+        if (low && tws < n_own)
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) sF[tws * DIM + k] = -(F[k] * chord);
     }
     __syncthreads();
 
