@@ -196,12 +196,17 @@ void Engine::upload() {
         // halo_off[b]) / 2), oriented for the slot that evaluates its flux:
         //   edge:   {ec xyz, |ec|}, {en xyz, chord}
         // Record bits: x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27;
-        // y = minus1 | (own: twin slot, halo: owner local) << 9 | low << 18 |
-        // edge << 19 | negate-en << 26.  An own slot finds its owner in LDS.
+        // y = minus1 | owner << 9 | edge << 18 | negate-en << 25 (face-set
+        // locals).  Per edge, edge_rec = evaluating slot | twin slot << 9 |
+        // (evaluating slot has the lower half-edge index) << 18.
         const size_t n_ext = ext_loc_.size() / 4;
         if (n_ext % 2) throw std::runtime_error("odd slot count");
+        // the kernel addresses every table with 32-bit byte offsets
+        if (16 * n_ext >= (size_t(1) << 32) || 8 * size_t(nf) * size_t(dim_) >= (size_t(1) << 32))
+            throw std::runtime_error("mesh too large for the mixed-precision path's 32-bit table offsets");
         std::vector<uint32_t> sa(4 * n_ext, 0u);
         std::vector<float> sb(4 * n_ext), ed(4 * n_ext);  // ed: n_ext/2 edges of 8 floats
+        std::vector<uint32_t> erec(n_ext / 2, 0u);
         const int32_t nb = int32_t(blk_face_.size()) - 1;
         std::vector<int32_t> eidx, own_of_halo;
         for (int32_t b = 0; b < nb; ++b) {
@@ -238,12 +243,12 @@ void Engine::upload() {
                     throw std::runtime_error("slot record does not fit its 9-bit local indices / 5-bit valence");
                 const bool own = s < n_own;
                 const bool rep = own && (low_of(s) || tw_of(s) >= n_own);
-                const uint32_t mid = own ? uint32_t(tw_of(s)) : owner;
-                const uint32_t low = own && low_of(s) ? 1u : 0u;
                 const uint32_t ei = uint32_t(eidx[size_t(s)]);
-                if (mid > 511 || ei > 127) throw std::runtime_error("slot record does not fit its fields");
+                if (ei > 127) throw std::runtime_error("slot record does not fit its fields");
                 sa[4 * e] = q[0] | q[1] << 9 | q[2] << 18 | valence << 27;
-                sa[4 * e + 1] = q[3] | mid << 9 | low << 18 | ei << 19 | (rep ? 0u : 1u) << 26;
+                sa[4 * e + 1] = q[3] | owner << 9 | ei << 18 | (rep ? 0u : 1u) << 25;
+                if (rep)
+                    erec[e0 / 2 + ei] = uint32_t(s) | uint32_t(tw_of(s)) << 9 | (low_of(s) ? 1u : 0u) << 18;
                 const float bp = float(beta[4 * hn + 0]), bm = float(beta[4 * hn + 2]);
                 std::memcpy(&sa[4 * e + 2], &bp, 4);
                 std::memcpy(&sa[4 * e + 3], &bm, 4);
@@ -266,6 +271,7 @@ void Engine::upload() {
         d.slot_a = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * sa.size(), sa.data()));
         d.slot_b = static_cast<const float*>(dev(sizeof(float) * sb.size(), sb.data()));
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * ed.size(), ed.data()));
+        d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * erec.size(), erec.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off_.size(), fset_off_.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face_.size(), fset_face_.data()));
         int32_t mx = 1;
@@ -275,7 +281,7 @@ void Engine::upload() {
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
-        d.slot_a = nullptr;
+        d.slot_a = d.edge_rec = nullptr;
         d.acc_rate = nullptr;
         d.fset_off = d.fset_face = nullptr;
         d.slot_b = d.edge32 = nullptr;

@@ -64,15 +64,17 @@ struct GeomPtrs {
     // halo_off[b]):
     //   slot_a [n_ext][4]: packed record x, y; beta+0, beta-0 (float bits)
     //     x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27
-    //     y = minus1 | (own: twin slot | halo: owner local) << 9 | low << 18 |
-    //         edge << 19 | negate-en << 26   (stencil fields: face-set locals)
+    //     y = minus1 | owner << 9 | edge << 18 | negate-en << 25
+    //     (face fields: face-set locals)
     //   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
     // and per edge of a block (n_slots/2 edges from row (foff[f0] +
     // halo_off[b]) / 2, oriented for the slot that evaluates the flux):
     //   edge32 [n_ext/2][8]: unit edge midpoint xyz, |ec|, normal xyz, chord
+    //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | low << 18
     const uint32_t* slot_a;
     const float* slot_b;
     const float* edge32;
+    const uint32_t* edge_rec;
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])

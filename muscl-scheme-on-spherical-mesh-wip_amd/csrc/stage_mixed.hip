@@ -34,10 +34,22 @@ __device__ __forceinline__ float m_tanh(float x) {
 
 static_assert(kBlock <= 512, "slot indices are packed in 9 bits");
 
+// Element `idx` of a device table through a 32-bit byte offset (each table
+// of the mixed path is below 4 GiB, checked at upload), so an access is a
+// scalar base plus one 32-bit vector offset instead of 64-bit address math.
+template <class T>
+__device__ __forceinline__ T* row(T* base, uint32_t idx) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * uint32_t(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const T* row(const T* base, uint32_t idx) {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * uint32_t(sizeof(T)));
+}
+
 template <int DIM>
 __device__ __forceinline__ void store_w32(float* __restrict__ W32, int nf, int i, const double* w) {
-    reinterpret_cast<float4*>(W32)[i] = make_float4(float(w[0]), float(w[1]), float(w[2]), float(w[3]));
-    if constexpr (DIM == 5) W32[4 * size_t(nf) + i] = float(w[4]);
+    *row(reinterpret_cast<float4*>(W32), uint32_t(i)) = make_float4(float(w[0]), float(w[1]), float(w[2]), float(w[3]));
+    if constexpr (DIM == 5) *row(W32, uint32_t(4 * nf + i)) = float(w[4]);
 }
 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
@@ -202,66 +214,60 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
     if (face) {
-        hb = g.foff[i];
-        he = g.foff[i + 1];
+        hb = *row(g.foff, uint32_t(i));
+        he = *row(g.foff, uint32_t(i + 1));
+        const double* ui = row(U, uint32_t(i * DIM));
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) u0[k] = U[size_t(i) * DIM + k];
-        const double4 c4 = reinterpret_cast<const double4*>(g.fcn)[i];
+        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
+        const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
         fcn = v3(c4.x, c4.y, c4.z);
-        area = g.area[i];
-        if (DIM == 5 && P.accretion_on) rate = g.acc_rate[i];
+        area = *row(g.area, uint32_t(i));
+        if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
     }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
-    // faces first, so face f0 + t sits at local index t) and each own slot's
-    // owner; the slot and edge rows are requested first so all loads overlap
+    // faces first, so face f0 + t sits at local index t); the slot and edge
+    // rows are requested first so all loads overlap
     extern __shared__ float4 sW[];
     float* sP = reinterpret_cast<float*>(sW + g.fset_max);
-    __shared__ uint8_t sOwn[kBlock];
     const bool own = t < n_own;
     const int ext = hbase + hoff + t;
     uint4 ra = make_uint4(0u, 0u, 0u, 0u);
     float4 hs, e4, n4;
     if (t < n_slots) {
-        ra = reinterpret_cast<const uint4*>(g.slot_a)[ext];
-        hs = reinterpret_cast<const float4*>(g.slot_b)[ext];
-        const int edge = ((hbase + hoff) >> 1) + int((ra.y >> 19) & 127u);
-        e4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge)];
-        n4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge) + 1];
+        ra = *row(reinterpret_cast<const uint4*>(g.slot_a), uint32_t(ext));
+        hs = *row(reinterpret_cast<const float4*>(g.slot_b), uint32_t(ext));
+        const uint32_t edge = uint32_t((hbase + hoff) >> 1) + ((ra.y >> 18) & 127u);
+        e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
+        n4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge + 1);
     }
     {
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
         const float4* W4 = reinterpret_cast<const float4*>(W32);
-        const float* Wp = W32 + 4 * size_t(g.nf);
         for (int q = t; q < n_fs; q += kBlock) {
-            const int f = g.fset_face[fs0 + q];
-            sW[q] = W4[f];
-            if constexpr (DIM == 5) sP[q] = Wp[f];
+            const uint32_t f = uint32_t(*row(g.fset_face, uint32_t(fs0 + q)));
+            sW[q] = *row(W4, f);
+            if constexpr (DIM == 5) sP[q] = *row(W32, uint32_t(4 * g.nf) + f);
         }
     }
-    if (face)
-        for (int hh = hb; hh < he; ++hh) sOwn[hh - hbase] = uint8_t(t);
-    const int mid = int((ra.y >> 9) & 511u);  // own: twin slot; halo: owner local
     __syncthreads();
 
-    // phase 1: reconstruct own and halo half-edges into LDS; the slot that
-    // evaluates an edge's flux records itself under the edge's index, and
-    // thread k requests the rows of edge k for phase 2
-    __shared__ uint32_t sRep[kBlock / 2];
+    // phase 1: reconstruct own and halo half-edges into LDS; thread k
+    // requests the rows of edge k for phase 2
     const int n_edges = n_slots >> 1;
     if (t < n_slots) {
-        const int li[5] = {own ? int(sOwn[t]) : mid, int(ra.x & 511u), int((ra.x >> 9) & 511u),
+        const int li[5] = {int((ra.y >> 9) & 511u), int(ra.x & 511u), int((ra.x >> 9) & 511u),
                            int((ra.x >> 18) & 511u), int(ra.y & 511u)};
-        const bool rep = !((ra.y >> 26) & 1u);
-        const float sg = rep ? 1.0f : -1.0f;
+        const float sg = (ra.y >> 25) & 1u ? -1.0f : 1.0f;
         reconstruct_lds<DIM>(P, float(dt), sW, sP, li, int(ra.x >> 27), __uint_as_float(ra.z),
                              __uint_as_float(ra.w), hs, e4, v3(sg * n4.x, sg * n4.y, sg * n4.z), &sUP[t * DIM]);
-        if (rep) sRep[(ra.y >> 19) & 127u] = uint32_t(t) | (ra.y & (1023u << 9));  // slot | twin slot, low
     }
+    uint32_t rec = 0;
     if (t < n_edges) {
-        const int edge = ((hbase + hoff) >> 1) + t;
-        e4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge)];
-        n4 = reinterpret_cast<const float4*>(g.edge32)[2 * size_t(edge) + 1];
+        const uint32_t edge = uint32_t(((hbase + hoff) >> 1) + t);
+        e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
+        n4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge + 1);
+        rec = *row(g.edge_rec, edge);
     }
     __syncthreads();
 
@@ -270,7 +276,6 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     // the negated normal) and written as +F*chord / -F*chord, so the two
     // faces of an edge exchange exactly opposite fluxes
     if (t < n_edges) {
-        const uint32_t rec = sRep[t];
         const int sw = int(rec & 511u);          // the evaluating (own) slot
         const int tws = int((rec >> 9) & 511u);  // its twin's slot
         const bool low = (rec >> 18) & 1u;
@@ -341,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
         store_w32<DIM>(W32_out, g.nf, i, wn);
         if constexpr (STAGE == 2) {
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) U[size_t(i) * DIM + k] = un[k];
+            for (int k = 0; k < DIM; ++k) row(U, uint32_t(i * DIM))[k] = un[k];
         }
     }
     if constexpr (STAGE == 2) {
