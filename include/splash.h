@@ -157,10 +157,11 @@ int splash_geometry_create(const splash_mesh_desc *mesh, int threads, splash_geo
 int splash_geometry_get_view(const splash_geometry *g, splash_geometry_view *out);
 void splash_geometry_free(splash_geometry *g);
 
-/* Host-side check of the internal GPU layout of a mesh (Morton face order,
- * 256-slot patches with their halos and LDS face sets): builds and
- * validates it and returns {blocks (multiple of 8), non-empty blocks, halo
- * half-edges, max halo, max face set}. */
+/* Host-side check of the internal GPU layout of a mesh that the
+ * mixed-precision kernel uses (Morton face order, patches of at most 64
+ * faces whose half-edges plus halo fit 512 LDS slots, LDS face sets):
+ * builds and validates it and returns {blocks (multiple of 8), non-empty
+ * blocks, halo half-edges, max halo, max face set}. */
 int splash_layout_stats(const splash_mesh_desc *mesh, int32_t out[5]);
 
 /* Human-readable text of the last error on this host thread. */

@@ -11,6 +11,7 @@
 
 #include "engine.hpp"
 #include "geometry.hpp"
+#include "launch.hpp"
 #include "layout.hpp"
 #include "mesh.hpp"
 
@@ -296,7 +297,7 @@ int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
     if (!out) return fail(SPLASH_EINVAL, "null output");
     return guarded([&] {
         const splash::Geometry g = splash::build_geometry(from_desc(mesh), 0);
-        const splash::Layout L = splash::build_layout(g, 256);
+        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(), splash::mixed::max_faces());
         splash::validate_layout(g, L);
         int32_t max_halo = 0, nonempty = 0;
         for (int32_t b = 0; b < L.n_blocks(); ++b) {

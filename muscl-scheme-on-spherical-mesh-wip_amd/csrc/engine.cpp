@@ -86,7 +86,8 @@ void Engine::upload() {
     const int32_t nf = g.nf, nh = g.nh;
     if (g.max_valence >= 32) throw std::runtime_error("face valence >= 32 is not supported");
     {
-        Layout L = build_layout(g, fp64::block_size());
+        Layout L = precision_ == kMixed ? build_layout(g, mixed::slots(), mixed::max_faces())
+                                        : build_layout(g, fp64::block_size(), fp64::block_size());
         validate_layout(g, L);
         perm_ = std::move(L.perm);
         iperm_ = std::move(L.iperm);
@@ -195,8 +196,8 @@ void Engine::upload() {
         // edges, so it has n_slots/2 edges, starting at row (foff[f0] +
         // halo_off[b]) / 2), oriented for the slot that evaluates its flux:
         //   edge:   {ec xyz, |ec|}, {en xyz, chord}
-        // Record bits: x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27;
-        // y = minus1 | owner << 9 | edge << 18 | negate-en << 25 (face-set
+        // Record bits: x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30;
+        // y = minus1 | owner << 10 | edge << 20 | valence << 28 (face-set
         // locals).  Per edge, edge_rec = evaluating slot | twin slot << 9 |
         // (evaluating slot has the lower half-edge index) << 18.
         const size_t n_ext = ext_loc_.size() / 4;
@@ -239,14 +240,14 @@ void Engine::upload() {
                 const uint32_t owner = uint32_t(L[0] & 0xffff), valence = uint32_t(L[0] >> 16);
                 const uint32_t q[4] = {uint32_t(L[1] & 0xffff), uint32_t(L[1] >> 16), uint32_t(L[2] & 0xffff),
                                        uint32_t(L[2] >> 16)};
-                if (owner > 511 || q[0] > 511 || q[1] > 511 || q[2] > 511 || q[3] > 511 || valence > 31)
-                    throw std::runtime_error("slot record does not fit its 9-bit local indices / 5-bit valence");
+                if (owner > 1023 || q[0] > 1023 || q[1] > 1023 || q[2] > 1023 || q[3] > 1023 || valence > 15)
+                    throw std::runtime_error("slot record does not fit its 10-bit local indices / 4-bit valence");
                 const bool own = s < n_own;
                 const bool rep = own && (low_of(s) || tw_of(s) >= n_own);
                 const uint32_t ei = uint32_t(eidx[size_t(s)]);
-                if (ei > 127) throw std::runtime_error("slot record does not fit its fields");
-                sa[4 * e] = q[0] | q[1] << 9 | q[2] << 18 | valence << 27;
-                sa[4 * e + 1] = q[3] | owner << 9 | ei << 18 | (rep ? 0u : 1u) << 25;
+                if (ei > 255 || n_slots > 512) throw std::runtime_error("slot record does not fit its fields");
+                sa[4 * e] = q[0] | q[1] << 10 | q[2] << 20 | (rep ? 0u : 1u) << 30;
+                sa[4 * e + 1] = q[3] | owner << 10 | ei << 20 | valence << 28;
                 if (rep)
                     erec[e0 / 2 + ei] = uint32_t(s) | uint32_t(tw_of(s)) << 9 | (low_of(s) ? 1u : 0u) << 18;
                 const float bp = float(beta[4 * hn + 0]), bm = float(beta[4 * hn + 2]);

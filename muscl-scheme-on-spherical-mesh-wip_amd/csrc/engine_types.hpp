@@ -63,8 +63,8 @@ struct GeomPtrs {
     // row (block b's own then halo half-edges start at foff[f0] +
     // halo_off[b]):
     //   slot_a [n_ext][4]: packed record x, y; beta+0, beta-0 (float bits)
-    //     x = plus0 | plus1 << 9 | minus0 << 18 | valence << 27
-    //     y = minus1 | owner << 9 | edge << 18 | negate-en << 25
+    //     x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30
+    //     y = minus1 | owner << 10 | edge << 20 | valence << 28
     //     (face fields: face-set locals)
     //   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
     // and per edge of a block (n_slots/2 edges from row (foff[f0] +

@@ -29,6 +29,8 @@ int block_size();
 
 namespace mixed {
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s);
+int slots();      // LDS half-edge slots per block
+int max_faces();  // faces per block
 }  // namespace mixed
 
 }  // namespace splash

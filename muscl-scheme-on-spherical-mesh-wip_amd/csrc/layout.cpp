@@ -25,10 +25,11 @@ void require(bool ok, const char* what) {
 
 }  // namespace
 
-Layout build_layout(const Geometry& g, int32_t slots) {
+Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     const int32_t nf = g.nf, nh = g.nh;
     Layout L;
     L.slots = slots;
+    L.max_faces = max_faces;
     std::vector<uint64_t> code(nf);
     for (int32_t f = 0; f < nf; ++f) {
         uint64_t c = 0;
@@ -59,6 +60,7 @@ Layout build_layout(const Geometry& g, int32_t slots) {
     auto twin_owner = [&](int32_t hn) { return L.iperm[g.he_face[g.twin[L.he_old[hn]]]]; };
 
     // greedy patches: add faces while own half-edges + halo fit the slots
+    // and the face count stays within max_faces
     L.blk_face.assign(1, 0);
     int32_t own = 0, halo = 0;
     for (int32_t k = 0; k < nf; ++k) {
@@ -70,7 +72,7 @@ Layout build_layout(const Geometry& g, int32_t slots) {
             const int32_t o = twin_owner(hn);
             dh += (o >= f0 && o < k) ? -1 : 1;
         }
-        if (own + val + halo + dh > slots) {
+        if (own + val + halo + dh > slots || k - f0 >= max_faces) {
             L.blk_face.push_back(k);
             own = 0;
             halo = 0;
@@ -156,7 +158,7 @@ void validate_layout(const Geometry& g, const Layout& L) {
     require(int32_t(L.fset_off.size()) == nb + 1 && L.n_ext() == nh + int32_t(L.halo_he.size()), "face sets");
     for (int32_t b = 0; b < nb; ++b) {
         const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
-        require(f0 <= f1 && f1 - f0 <= L.slots, "faces per block <= threads");
+        require(f0 <= f1 && f1 - f0 <= L.max_faces, "faces per block within the limit");
         const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
         const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
         require(n_halo >= 0 && n_own + n_halo <= L.slots, "own + halo half-edges fit the LDS slots");

@@ -14,6 +14,7 @@ namespace splash {
 
 struct Layout {
     int32_t slots = 0;                 // LDS half-edge slots per block
+    int32_t max_faces = 0;             // faces per block (one thread per face in the update)
     std::vector<int32_t> perm, iperm;  // internal -> mesh face, mesh -> internal
     std::vector<int32_t> foff;         // [nf+1] internal face offsets
     std::vector<int32_t> he_old;       // internal -> mesh half-edge
@@ -26,7 +27,7 @@ struct Layout {
     // fset_off[b+1]) (its own faces first, then every other face its own and
     // halo half-edges read); slot rows of block b start at foff[f0] +
     // halo_off[b] ("extended rows", own then halo, nh + halo_he.size() rows)
-    static constexpr int32_t kFaceSetMax = 512;
+    static constexpr int32_t kFaceSetMax = 1024;
     std::vector<int32_t> fset_off;     // [n_blocks+1]
     std::vector<int32_t> fset_face;    // internal face ids
     std::vector<int32_t> ext_loc;      // [n_ext][4]: owner local | valence << 16, plus0 | plus1 << 16,
@@ -35,7 +36,9 @@ struct Layout {
     int32_t n_ext() const { return int32_t(ext_loc.size() / 4); }
 };
 
-Layout build_layout(const Geometry& g, int32_t slots);
+// Patches of at most max_faces faces whose own half-edges plus halo fit in
+// `slots` LDS slots.
+Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces);
 
 // Checks every index the stage kernel dereferences (throws std::logic_error
 // on the first violation).

@@ -3,6 +3,8 @@
 // reductions and geometry in FP64.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "engine_types.hpp"
 #include "launch.hpp"
 
@@ -32,7 +34,11 @@ __device__ __forceinline__ float m_tanh(float x) {
 // slot.  Face areas, the state, the source terms, the dt reductions and the
 // flux accumulation (compensated, Neumaier) stay FP64.
 
-static_assert(kBlock <= 512, "slot indices are packed in 9 bits");
+// Patches: at most kFaces faces (one wave updates them) whose own
+// half-edges plus halo fit kSlots LDS slots, two slots per thread.
+constexpr int kSlots = 2 * kBlock;
+constexpr int kFaces = 64;
+static_assert(kSlots <= 512, "slot indices are packed in 9 bits");
 
 // Element `idx` of a device table through a 32-bit byte offset (each table
 // of the mixed path is below 4 GiB, checked at upload), so an access is a
@@ -192,12 +198,43 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
     if (cs > vmax) vmax = cs;
 }
 
+// One slot's rows: packed record (x, y, beta+0, beta-0), {1/H+, 1/H-, BM,
+// arc/area} and its edge's {ec, |ec|}, {en, chord}.
+struct SlotRows {
+    uint4 ra;
+    float4 hs, e4, n4;
+};
+
+__device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, uint32_t edge_base) {
+    SlotRows r;
+    r.ra = *row(reinterpret_cast<const uint4*>(g.slot_a), ext);
+    r.hs = *row(reinterpret_cast<const float4*>(g.slot_b), ext);
+    const uint32_t edge = edge_base + ((r.ra.y >> 20) & 255u);
+    r.e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
+    r.n4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge + 1);
+    return r;
+}
+
+// Record bits (face fields are face-set locals): x = plus0 | plus1 << 10 |
+// minus0 << 20 | negate-en << 30; y = minus1 | owner << 10 | edge << 20 |
+// valence << 28.
+template <int DIM>
+__device__ __forceinline__ void reconstruct_slot(const PhysParams& P, float dt, const float4* sW, const float* sP,
+                                                 const SlotRows& r, float* out) {
+    const uint4 ra = r.ra;
+    const int li[5] = {int((ra.y >> 10) & 1023u), int(ra.x & 1023u), int((ra.x >> 10) & 1023u),
+                       int((ra.x >> 20) & 1023u), int(ra.y & 1023u)};
+    const float sg = (ra.x >> 30) & 1u ? -1.0f : 1.0f;
+    reconstruct_lds<DIM>(P, dt, sW, sP, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w), r.hs,
+                         r.e4, v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
+}
+
 template <int DIM, int SOLVER, int STAGE>
-__global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Scalars* __restrict__ sc,
+__global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P, Scalars* __restrict__ sc,
                                                     double* __restrict__ U, const float* __restrict__ W32,
                                                     float* __restrict__ W32_out) {
-    __shared__ float sUP[kBlock * DIM];
-    __shared__ float sF[kBlock * DIM];  // edge flux times chord length, per own slot
+    __shared__ float sUP[kSlots * DIM];
+    __shared__ float sF[kSlots * DIM];  // edge flux times chord length, per own slot
     const int b = xcd_block();
     const int f0 = g.blk_face[b], f1 = g.blk_face[b + 1];
     const int hbase = g.foff[f0];
@@ -206,10 +243,11 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     const int n_slots = n_own + (g.halo_off[b + 1] - hoff);
     const int t = threadIdx.x;
     const double dt = sc->dt;
+    const uint32_t ext0 = uint32_t(hbase + hoff), edge_base = ext0 >> 1;
 
     // face data of phase 3, requested before the stencil gathers
     const int i = f0 + t;
-    const bool face = i < f1;
+    const bool face = i < f1;  // t < kFaces
     double u0[DIM], area = 1, rate = 0;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
@@ -226,21 +264,12 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
-    // faces first, so face f0 + t sits at local index t); the slot and edge
-    // rows are requested first so all loads overlap
+    // faces first, so face f0 + t sits at local index t); the first slot's
+    // rows are requested first so the loads overlap
     extern __shared__ float4 sW[];
     float* sP = reinterpret_cast<float*>(sW + g.fset_max);
-    const bool own = t < n_own;
-    const int ext = hbase + hoff + t;
-    uint4 ra = make_uint4(0u, 0u, 0u, 0u);
-    float4 hs, e4, n4;
-    if (t < n_slots) {
-        ra = *row(reinterpret_cast<const uint4*>(g.slot_a), uint32_t(ext));
-        hs = *row(reinterpret_cast<const float4*>(g.slot_b), uint32_t(ext));
-        const uint32_t edge = uint32_t((hbase + hoff) >> 1) + ((ra.y >> 18) & 127u);
-        e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
-        n4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge + 1);
-    }
+    SlotRows r0;
+    if (t < n_slots) r0 = load_slot(g, ext0 + uint32_t(t), edge_base);
     {
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
         const float4* W4 = reinterpret_cast<const float4*>(W32);
@@ -252,19 +281,20 @@ __global__ __launch_bounds__(kBlock) void k_stage32(GeomPtrs g, PhysParams P, Sc
     }
     __syncthreads();
 
-    // phase 1: reconstruct own and halo half-edges into LDS; thread k
-    // requests the rows of edge k for phase 2
+    // phase 1: reconstruct own and halo half-edges into LDS, slots t and
+    // t + kBlock; thread k then requests the rows of edge k for phase 2
     const int n_edges = n_slots >> 1;
-    if (t < n_slots) {
-        const int li[5] = {int((ra.y >> 9) & 511u), int(ra.x & 511u), int((ra.x >> 9) & 511u),
-                           int((ra.x >> 18) & 511u), int(ra.y & 511u)};
-        const float sg = (ra.y >> 25) & 1u ? -1.0f : 1.0f;
-        reconstruct_lds<DIM>(P, float(dt), sW, sP, li, int(ra.x >> 27), __uint_as_float(ra.z),
-                             __uint_as_float(ra.w), hs, e4, v3(sg * n4.x, sg * n4.y, sg * n4.z), &sUP[t * DIM]);
+    {
+        const int t1 = t + kBlock;
+        SlotRows r1;
+        if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
+        if (t < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, r0, &sUP[t * DIM]);
+        if (t1 < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, r1, &sUP[t1 * DIM]);
     }
     uint32_t rec = 0;
+    float4 e4, n4;
     if (t < n_edges) {
-        const uint32_t edge = uint32_t(((hbase + hoff) >> 1) + t);
+        const uint32_t edge = edge_base + uint32_t(t);
         e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
         n4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge + 1);
         rec = *row(g.edge_rec, edge);
@@ -363,6 +393,8 @@ namespace {
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s) {
     const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 20 : 16);
+    if (a.geom.fset_max > 1024 || lds + 2 * sizeof(float) * kSlots * DIM > 64 * 1024)
+        throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
         k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b);
     else
@@ -370,6 +402,9 @@ void stage(const StepArgs& a, int stage, hipStream_t s) {
 }
 
 }  // namespace
+
+int slots() { return kSlots; }
+int max_faces() { return kFaces; }
 
 void enqueue_stage(const StepArgs& a, int st, hipStream_t s) {
     switch (a.phys.solver) {

@@ -83,6 +83,7 @@ def test_gpu_layout_validates_on_every_mesh_family():
     for mesh in (S.icosphere_hex(5), S.quad_sphere(5), S.uv_sphere(96, 48), S.icosphere(4), S.icosphere_hex(0)):
         blocks, nonempty, halo, max_halo, max_fset = S.layout_stats(mesh)
         assert blocks % 8 == 0 and 0 < nonempty <= blocks
-        assert 0 <= halo < mesh.n_halfedges and max_halo < 256
+        assert 0 <= halo < mesh.n_halfedges and max_halo < 512
         assert (halo > 0) == (nonempty > 1)
-        assert 0 < max_fset <= 512
+        assert 0 < max_fset <= 1024
+        assert nonempty >= mesh.n_faces / 64
