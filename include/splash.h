@@ -106,8 +106,9 @@ void splash_destroy(splash_ctx *ctx);
 int splash_do_step(splash_ctx *ctx, double dt0);
 /* n consecutive do_step(dt0) calls, replayed from a captured HIP graph. */
 int splash_do_steps(splash_ctx *ctx, double dt0, int n);
-/* Measurement extension: runs n eager do_step(dt0) calls with HIP events
- * around every launch on the context's stream; ms_out[4] receives the mean
+/* Measurement extension: runs n do_step(dt0) calls back to back on the
+ * context's stream, each kernel launched with start/stop HIP events bound
+ * to its dispatch, one wait at the end; ms_out[4] receives the mean
  * duration in ms of [dt finalize, stage 1, stage 2, whole step]. */
 int splash_profile_step(splash_ctx *ctx, double dt0, int n, double *ms_out);
 /* Waits for every enqueued step. */

@@ -322,10 +322,10 @@ void Engine::upload() {
     d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
 }
 
-void Engine::enqueue_kernel(int which, double dt0) {
-    if (which == 0) fp64::enqueue_finalize_dt(args_, dt0, stream_);
-    else if (precision_ == kFP64) fp64::enqueue_stage(args_, which, stream_);
-    else mixed::enqueue_stage(args_, which, stream_);
+void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev) {
+    if (which == 0) fp64::enqueue_finalize_dt(args_, dt0, stream_, ev);
+    else if (precision_ == kFP64) fp64::enqueue_stage(args_, which, stream_, ev);
+    else mixed::enqueue_stage(args_, which, stream_, ev);
 }
 
 void Engine::enqueue_step(double dt0) {
@@ -333,33 +333,36 @@ void Engine::enqueue_step(double dt0) {
 }
 
 void Engine::profile(double dt0, int n, double* ms) {
-    // HIP events around every launch of n eager steps on this run's stream
-    std::vector<hipEvent_t> ev(kKernelsPerStep + 1);
-    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
-    double acc[kKernelsPerStep + 1] = {0};
+    // n steps launched back to back on this run's stream, each kernel with
+    // start/stop HIP events bound to its own dispatch, one wait at the end
+    constexpr int K = kKernelsPerStep;
+    const size_t nk = size_t(n > 0 ? n : 0) * K;
+    std::vector<hipEvent_t> ev(2 * nk, nullptr);
+    double acc[K + 1] = {0};
     try {
-        for (int s = 0; s < n; ++s) {
-            check(hipEventRecord(ev[0], stream_), "hipEventRecord");
-            for (int k = 0; k < kKernelsPerStep; ++k) {
-                enqueue_kernel(k, dt0);
-                check(hipEventRecord(ev[k + 1], stream_), "hipEventRecord");
-            }
-            check(hipEventSynchronize(ev[kKernelsPerStep]), "hipEventSynchronize");
-            for (int k = 0; k < kKernelsPerStep; ++k) {
-                float t = 0;
-                check(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "hipEventElapsedTime");
-                acc[k] += t;
-            }
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        for (size_t q = 0; q < nk; ++q) {
+            const KernelEvents ke{ev[2 * q], ev[2 * q + 1]};
+            enqueue_kernel(int(q % K), dt0, &ke);
+        }
+        check(hipGetLastError(), "profile launch");
+        if (nk) check(hipEventSynchronize(ev.back()), "hipEventSynchronize");
+        for (size_t q = 0; q < nk; ++q) {
             float t = 0;
-            check(hipEventElapsedTime(&t, ev[0], ev[kKernelsPerStep]), "hipEventElapsedTime");
-            acc[kKernelsPerStep] += t;
+            check(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]), "hipEventElapsedTime");
+            acc[q % K] += t;
+            if (q % K == K - 1) {
+                check(hipEventElapsedTime(&t, ev[2 * (q - K + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
+                acc[K] += t;
+            }
         }
     } catch (...) {
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
         throw;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k <= kKernelsPerStep; ++k) ms[k] = n > 0 ? acc[k] / n : 0.0;
+    for (int k = 0; k <= K; ++k) ms[k] = n > 0 ? acc[k] / n : 0.0;
 }
 
 void Engine::step(double dt0) {

@@ -70,7 +70,7 @@ public:
 
 private:
     void enqueue_step(double dt0);
-    void enqueue_kernel(int which, double dt0);
+    void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr);
     void upload();
     Scalars read_scalars();
 

@@ -19,16 +19,22 @@ struct StepArgs {
     float* W32b; // mixed runs: the other stage's FP32 input
 };
 
+// Optional start/stop events of one launch: bound to the kernel's own
+// dispatch (hipExtLaunchKernel), so they time the kernel alone.
+struct KernelEvents {
+    hipEvent_t start, stop;
+};
+
 namespace fp64 {
 void enqueue_prepare(const StepArgs& a, hipStream_t s);
-void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s);
-void enqueue_stage(const StepArgs& a, int stage, hipStream_t s);
+void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev = nullptr);
+void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
 void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s);
 int block_size();
 }  // namespace fp64
 
 namespace mixed {
-void enqueue_stage(const StepArgs& a, int stage, hipStream_t s);
+void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
 int slots();      // LDS half-edge slots per block
 int max_faces();  // faces per block
 }  // namespace mixed

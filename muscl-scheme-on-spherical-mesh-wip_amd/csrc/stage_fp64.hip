@@ -1,6 +1,7 @@
 // FP64 instantiation of the stage kernels.  This translation unit is built
 // with -ffp-contract=off: its arithmetic follows the reference's operation
 // order so the plain FP64 path can be compared with the CPU restatement.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "engine_types.hpp"
@@ -13,21 +14,23 @@ namespace fp64 {
 namespace {
 
 template <int DIM, int SOLVER>
-void stage(const StepArgs& a, int stage, hipStream_t s) {
+void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
     if (stage == 1)
-        k_stage<double, DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W, a.W2);
+        launch(k_stage<double, DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, 0, s, ev, a.geom, a.phys, a.sc, a.U, a.W,
+               a.W2);
     else
-        k_stage<double, DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, a.W2, a.W);
+        launch(k_stage<double, DIM, SOLVER, 2>, a.geom.n_blocks, kBlock, 0, s, ev, a.geom, a.phys, a.sc, a.U, a.W2,
+               a.W);
 }
 
 }  // namespace
 
-void enqueue_stage(const StepArgs& a, int st, hipStream_t s) {
+void enqueue_stage(const StepArgs& a, int st, hipStream_t s, const KernelEvents* ev) {
     switch (a.phys.solver) {
-    case kHLLE: stage<4, kHLLE>(a, st, s); break;
-    case kHLLE_P: stage<5, kHLLE_P>(a, st, s); break;
-    case kHLLC: stage<5, kHLLC>(a, st, s); break;
-    default: stage<5, kHLLCPLUS>(a, st, s); break;
+    case kHLLE: stage<4, kHLLE>(a, st, s, ev); break;
+    case kHLLE_P: stage<5, kHLLE_P>(a, st, s, ev); break;
+    case kHLLC: stage<5, kHLLC>(a, st, s, ev); break;
+    default: stage<5, kHLLCPLUS>(a, st, s, ev); break;
     }
 }
 
@@ -36,9 +39,9 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s) {
     else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W, a.W32);
 }
 
-void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s) {
-    if (a.phys.dim == 5) k_finalize_dt<5><<<1, kFinalizeThreads, 0, s>>>(a.geom, a.phys, a.sc, dt0);
-    else k_finalize_dt<4><<<1, kFinalizeThreads, 0, s>>>(a.geom, a.phys, a.sc, dt0);
+void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev) {
+    if (a.phys.dim == 5) launch(k_finalize_dt<5>, 1, kFinalizeThreads, 0, s, ev, a.geom, a.phys, a.sc, dt0);
+    else launch(k_finalize_dt<4>, 1, kFinalizeThreads, 0, s, ev, a.geom, a.phys, a.sc, dt0);
 }
 
 

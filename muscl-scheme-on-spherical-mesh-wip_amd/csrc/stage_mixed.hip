@@ -1,6 +1,7 @@
 // Mixed-precision instantiation of the stage kernels: reconstruction slopes,
 // limiter and Riemann fluxes in FP32; state, RK combine, sources, dt
 // reductions and geometry in FP64.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -391,14 +392,16 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
 namespace {
 
 template <int DIM, int SOLVER>
-void stage(const StepArgs& a, int stage, hipStream_t s) {
+void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
     const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 20 : 16);
     if (a.geom.fset_max > 1024 || lds + 2 * sizeof(float) * kSlots * DIM > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
-        k_stage32<DIM, SOLVER, 1><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32, a.W32b);
+        launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys, a.sc, a.U, a.W32,
+               a.W32b);
     else
-        k_stage32<DIM, SOLVER, 2><<<a.geom.n_blocks, kBlock, lds, s>>>(a.geom, a.phys, a.sc, a.U, a.W32b, a.W32);
+        launch(k_stage32<DIM, SOLVER, 2>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys, a.sc, a.U, a.W32b,
+               a.W32);
 }
 
 }  // namespace
@@ -406,12 +409,12 @@ void stage(const StepArgs& a, int stage, hipStream_t s) {
 int slots() { return kSlots; }
 int max_faces() { return kFaces; }
 
-void enqueue_stage(const StepArgs& a, int st, hipStream_t s) {
+void enqueue_stage(const StepArgs& a, int st, hipStream_t s, const KernelEvents* ev) {
     switch (a.phys.solver) {
-    case kHLLE: stage<4, kHLLE>(a, st, s); break;
-    case kHLLE_P: stage<5, kHLLE_P>(a, st, s); break;
-    case kHLLC: stage<5, kHLLC>(a, st, s); break;
-    default: stage<5, kHLLCPLUS>(a, st, s); break;
+    case kHLLE: stage<4, kHLLE>(a, st, s, ev); break;
+    case kHLLE_P: stage<5, kHLLE_P>(a, st, s, ev); break;
+    case kHLLC: stage<5, kHLLC>(a, st, s, ev); break;
+    default: stage<5, kHLLCPLUS>(a, st, s, ev); break;
     }
 }
 

@@ -161,3 +161,21 @@ def test_c2_full_size_fp64_parity_and_conservation():
     d0 = ref.diagnostics()
     d = gpu.diagnostics()
     assert np.allclose(d, d0, rtol=1e-13, atol=1e-15)
+
+
+def test_profile_step_times_back_to_back_steps():
+    """profile_step advances exactly n steps, gives positive per-kernel means
+    that add up to the step time, and leaves the state where as many plain
+    steps leave it."""
+    import splash_amd as S
+    mesh = S.icosphere_hex(4)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=5, omega=(0, 0, 0.2))
+    for prec in (S.FP64, S.MIXED):
+        a = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
+        b = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
+        ms = a.profile_step(1e-4, 5)
+        for _ in range(5):
+            b.do_step(1e-4)
+        assert a.time() == b.time()
+        assert np.array_equal(a.get_state(), b.get_state())
+        assert np.all(ms > 0) and ms[:3].sum() <= ms[3] * 1.0001
