@@ -381,10 +381,26 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
         }
     }
     if constexpr (STAGE == 2) {
-        reduce_dt_limits(g.dt_partial, vmax, dtx);
-        if (blockIdx.x == 0 && t == 0) {
-            sc->t += sc->dt;
-            sc->steps += 1;
+        // this block's partial of max(|v|, c_s) and min friction dt: its faces
+        // are all in wave 0 (kFaces = 64), so the reduction stays in that
+        // wave and the other waves are already done
+        static_assert(kFaces == 64, "one wave holds a patch's faces");
+        if (t < 64) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(vmax, o);
+                const double ox = __shfl_xor(dtx, o);
+                if (ov > vmax) vmax = ov;
+                if (dtx > ox) dtx = ox;
+            }
+            if (t == 0) {
+                g.dt_partial[2 * blockIdx.x] = vmax;
+                g.dt_partial[2 * blockIdx.x + 1] = dtx;
+                if (blockIdx.x == 0) {
+                    sc->t += sc->dt;
+                    sc->steps += 1;
+                }
+            }
         }
     }
 }
