@@ -269,6 +269,9 @@ void Engine::upload() {
         std::vector<double> rate(nf);
         for (int32_t k = 0; k < nf; ++k) rate[k] = acc[8 * size_t(k)];
         d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
+        std::vector<double> rda(nf);
+        for (int32_t k = 0; k < nf; ++k) rda[k] = g.round_diff / area[k];
+        d.rd_area = static_cast<const double*>(dev(sizeof(double) * rda.size(), rda.data()));
         d.slot_a = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * sa.size(), sa.data()));
         d.slot_b = static_cast<const float*>(dev(sizeof(float) * sb.size(), sb.data()));
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * ed.size(), ed.data()));
@@ -283,7 +286,7 @@ void Engine::upload() {
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     } else {
         d.slot_a = d.edge_rec = nullptr;
-        d.acc_rate = nullptr;
+        d.acc_rate = d.rd_area = nullptr;
         d.fset_off = d.fset_face = nullptr;
         d.slot_b = d.edge32 = nullptr;
         d.fset_max = 0;
@@ -308,6 +311,12 @@ void Engine::upload() {
     P.om_acc[0] = 0;
     P.om_acc[1] = std::sin(params_.tilt_angle) * params_.omega_acc_abs;
     P.om_acc[2] = std::cos(params_.tilt_angle) * params_.omega_acc_abs;
+    {
+        const double gam3d = 1 / (2 - gam_);
+        P.gam_h = gam_ / (gam_ - 1);
+        P.gam3d_rho = gam3d / (2 * gam3d - 1);
+        P.cool_k = 12. / 5 * 1.6e-13 * std::pow(3. / 4 * 4.85e36, 0.25);  // adiabatic.hpp cooling constants
+    }
 
     args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
     args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));

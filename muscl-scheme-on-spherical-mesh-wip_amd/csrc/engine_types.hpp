@@ -37,6 +37,10 @@ struct PhysParams {
     double e_acc, alpha;
     double dmdt_coef;   // -(fall_eff*acc_rate*4*pi)/total_mass (adiabatic.hpp:463)
     double om_acc[3];   // accretion angular velocity (0, sin(tilt), cos(tilt)) * omega_acc_abs
+    // constants of the mixed-precision kernel's FP32 source terms
+    double gam_h;       // gam / (gam - 1)
+    double gam3d_rho;   // gam3d / (2 gam3d - 1), gam3d = 1 / (2 - gam)
+    double cool_k;      // 12/5 k_m (3/4 c_sigma)^(1/4) of the cooling coefficient C
 };
 
 // Device tables in the internal face order (faces sorted along a Morton
@@ -78,6 +82,7 @@ struct GeomPtrs {
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
+    const double* rd_area;    // [nf] round_diff / area
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt

@@ -32,8 +32,9 @@ __device__ __forceinline__ float m_tanh(float x) {
 // slots and face loop) but works on FP32 copies: the stage input as one
 // float4 (Sigma, L) row per face plus (dim 5) one pressure per face, staged
 // per block in LDS, and four float4 rows plus one packed 8-byte record per
-// slot.  Face areas, the state, the source terms, the dt reductions and the
-// flux accumulation (compensated, Neumaier) stay FP64.
+// slot.  The state, the RK combine, the dt limits and reductions and the
+// flux accumulation (compensated, Neumaier) stay FP64; the accretion /
+// cooling / friction sources run in FP32 on the FP32 stage input.
 
 // Patches: at most kFaces faces (one wave updates them) whose own
 // half-edges plus halo fit kSlots LDS slots, two slots per thread.
@@ -66,8 +67,8 @@ __device__ __forceinline__ void store_w32(float* __restrict__ W32, int nf, int i
 // {ec, |ec|}; en = the half-edge's outward normal.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, const float* sP,
-                                                const int* li, int valence, float bp, float bm, float4 hs, float4 e4,
-                                                V3<float> en, float* out) {
+                                                const float* sC, const int* li, int valence, float bp, float bm,
+                                                float4 hs, float4 e4, V3<float> en, float* out) {
     float w[5][DIM];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -87,7 +88,7 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
     const float eta_m = __builtin_amdgcn_rcpf(hs.z * hs.y), eta_p = __builtin_amdgcn_rcpf(hs.z * hs.x);  // H-+/BM
     if constexpr (DIM == 5) {
         const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0] * e4.w);
-        const float cs = m_sqrt(m_div(float(P.gam) * wi[4], wi[0]));
+        const float cs = sC[li[0]];  // the owner's sound speed, staged with its state
         const float nu = (cs + dot(vel, en)) * dt * hs.w;
         limiter_adiabatic<float, DIM>(r, nu, eta_m, eta_p, valence, lim);
     } else {
@@ -104,66 +105,68 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
     }
 }
 
-// The source terms of adiabatic.hpp:396-541 (as source_adiabatic) in FP64
-// on the stage input's p-form state w, with the accretion vectors
-// omega_acc x r and r x (omega_acc x r) formed from the face centre, one
-// reciprocal per density / pressure instead of a division per use, and
-// x^(1/4) = sqrt(sqrt(x)), y^4 = (y^2)^2 in place of pow.
-__device__ __forceinline__ void source_mixed(const PhysParams& P, const double* w, V3<double> fcn, double rate,
-                                             double* res) {
-    const double gam = P.gam;
-    const double inv0 = 1.0 / w[0], invp = 1.0 / w[4];
-    const V3<double> L = v3(w[1], w[2], w[3]);
-    const V3<double> vel = cross(fcn, L) * (-inv0);
-    const V3<double> omxr = cross(v3(P.om_acc[0], P.om_acc[1], P.om_acc[2]), fcn);
-    const V3<double> rxo = cross(fcn, omxr);
+// The source terms of adiabatic.hpp:396-541 (as source_adiabatic) in FP32
+// on the face's FP32 stage input (Sigma, L, p): accretion with omega_acc x r
+// and r x (omega_acc x r) formed from the face centre, radiative cooling and
+// friction.  Cooling uses C = cool_k Sigma/p (g_eff Sigma)^(1/4), the
+// reference's 12/5 k_m Sigma/p (3/4 c_sigma g_eff Sigma)^(1/4) with the
+// constant factor taken out so nothing leaves FP32 range, and works with
+// 1 - beta directly (the reference clamps beta to [1e-8, 1 - 1e-8] and maps
+// NaN to 1e-8; in FP32 1 - 1e-8 rounds to 1).  Each term is scaled by dt
+// (<= 1e-4) and added to the FP64 state, so FP32 rounding enters at ~1e-11
+// of the state per step.
+__device__ __forceinline__ void source32(const PhysParams& P, float4 x, float p, V3<float> r, float rate,
+                                         float* res) {
+    const float sig = x.x;
+    const float inv0 = __builtin_amdgcn_rcpf(sig), invp = __builtin_amdgcn_rcpf(p);
+    const V3<float> L = v3(x.y, x.z, x.w);
+    const V3<float> vel = cross(r, L) * (-inv0);
+    const V3<float> omxr = cross(v3(float(P.om_acc[0]), float(P.om_acc[1]), float(P.om_acc[2])), r);
+    const V3<float> rxo = cross(r, omxr);
     res[0] = rate;
     res[1] = rate * rxo.x;
     res[2] = rate * rxo.y;
     res[3] = rate * rxo.z;
-    const V3<double> dvv = omxr - vel;
-    res[4] = rate * (P.e_acc + dot(dvv, dvv) * 0.5);
-    const double vn2 = dot(vel, vel);
-    const double h_spec = gam / (gam - 1) * w[4] * inv0;  // gam/(gam-1) p/Sigma
+    const V3<float> dvv = omxr - vel;
+    res[4] = rate * (float(P.e_acc) + dot(dvv, dvv) * 0.5f);
+    const float vn2 = dot(vel, vel);
+    const float h_spec = float(P.gam_h) * p * inv0;  // gam/(gam-1) p/Sigma
     if (!P.friction_on) {
-        const V3<double> rxv = cross(fcn, vel);
-        const double dmdt = P.dmdt_coef * w[0];
+        const V3<float> rxv = cross(r, vel);
+        const float dmdt = float(P.dmdt_coef) * sig;
         res[0] += dmdt;
         res[1] += dmdt * rxv.x;
         res[2] += dmdt * rxv.y;
         res[3] += dmdt * rxv.z;
-        res[4] += dmdt * (h_spec + vn2 * 0.5);
+        res[4] += dmdt * (h_spec + vn2 * 0.5f);
     }
-    const double g_eff = kGM - vn2;
-    const double c_sigma = 4.85e36, k_m = 1.6e-13, kappa = 3.4e6;
-    const double C = (12. / 5 * k_m) * w[0] * invp * sqrt(sqrt((3. / 4 * c_sigma) * g_eff * w[0]));
-    const double beta_switch = 0.736194670678821;
-    const double C_switch = -1 - 1 / (beta_switch - 1);
-    double beta;
+    const float g_eff = float(kGM) - vn2;
+    const float C = float(P.cool_k) * sig * invp * m_sqrt(m_sqrt(g_eff * sig));
+    constexpr float beta_switch = 0.736194670678821f;
+    constexpr float C_switch = -1 - 1 / (beta_switch - 1);
+    float omb;  // 1 - beta
     if (C <= C_switch) {
-        beta = 1 - 1 / (1 + C);
+        omb = __builtin_amdgcn_rcpf(1 + C);
     } else {
-        const double q = 2 / C, q2 = q * q;
-        beta = 1 - q2 * q2;
+        const float q = 2 * __builtin_amdgcn_rcpf(C), q2 = q * q;
+        omb = q2 * q2;
     }
-    const double beta_ceil = 1 - 1e-8, beta_floor = 1e-8;
-    if (beta < beta_floor || __builtin_isnan(beta)) beta = beta_floor;
-    if (beta > beta_ceil) beta = beta_ceil;
-    res[4] -= g_eff * (1 / kappa) * (1 - beta);
+    if (!(omb <= 1 - 1e-8f)) omb = 1 - 1e-8f;  // beta below the floor, or NaN
+    if (omb < 1e-8f) omb = 1e-8f;              // beta above the ceiling
+    res[4] -= g_eff * float(1 / 3.4e6) * omb;  // kappa
     if (P.friction_on) {
-        const double gam3d = 1 / (2 - gam);
-        const double rho = gam3d / (2 * gam3d - 1) * g_eff * w[0] * w[0] * invp;
-        const V3<double> vel0 = cross(v3(0.0, 0.0, P.omega_ns), fcn);
-        const V3<double> dv0 = vel - vel0;
-        const double a = P.alpha * rho * sqrt(dot(dv0, dv0));
-        const V3<double> wr = dv0 * (-a);
-        const V3<double> l_fr = cross(fcn, wr);
-        const double a0 = a * inv0;
+        const float rho = float(P.gam3d_rho) * g_eff * sig * sig * invp;
+        const V3<float> vel0 = cross(v3(0.0f, 0.0f, float(P.omega_ns)), r);
+        const V3<float> dv0 = vel - vel0;
+        const float a = float(P.alpha) * rho * m_sqrt(dot(dv0, dv0));
+        const V3<float> wr = dv0 * (-a);
+        const V3<float> l_fr = cross(r, wr);
+        const float a0 = a * inv0;
         res[0] -= a;
-        res[1] += l_fr.x - a0 * w[1];
-        res[2] += l_fr.y - a0 * w[2];
-        res[3] += l_fr.z - a0 * w[3];
-        res[4] += dot(wr, vel0) - a * (vn2 * 0.5 + h_spec);
+        res[1] += l_fr.x - a0 * L.x;
+        res[2] += l_fr.y - a0 * L.y;
+        res[3] += l_fr.z - a0 * L.z;
+        res[4] += dot(wr, vel0) - a * (vn2 * 0.5f + h_spec);
     }
 }
 
@@ -221,13 +224,13 @@ __device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, u
 // valence << 28.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_slot(const PhysParams& P, float dt, const float4* sW, const float* sP,
-                                                 const SlotRows& r, float* out) {
+                                                 const float* sC, const SlotRows& r, float* out) {
     const uint4 ra = r.ra;
     const int li[5] = {int((ra.y >> 10) & 1023u), int(ra.x & 1023u), int((ra.x >> 10) & 1023u),
                        int((ra.x >> 20) & 1023u), int(ra.y & 1023u)};
     const float sg = (ra.x >> 30) & 1u ? -1.0f : 1.0f;
-    reconstruct_lds<DIM>(P, dt, sW, sP, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w), r.hs,
-                         r.e4, v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
+    reconstruct_lds<DIM>(P, dt, sW, sP, sC, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w),
+                         r.hs, r.e4, v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
 }
 
 template <int DIM, int SOLVER, int STAGE>
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
     // face data of phase 3, requested before the stencil gathers
     const int i = f0 + t;
     const bool face = i < f1;  // t < kFaces
-    double u0[DIM], area = 1, rate = 0;
+    double u0[DIM], rd_area = 0, rate = 0;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
     if (face) {
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
         for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
         const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
         fcn = v3(c4.x, c4.y, c4.z);
-        area = *row(g.area, uint32_t(i));
+        rd_area = *row(g.rd_area, uint32_t(i));
         if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
     }
 
@@ -268,7 +271,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
     // faces first, so face f0 + t sits at local index t); the first slot's
     // rows are requested first so the loads overlap
     extern __shared__ float4 sW[];
-    float* sP = reinterpret_cast<float*>(sW + g.fset_max);
+    float* sP = reinterpret_cast<float*>(sW + g.fset_max);  // dim 5: pressure
+    float* sC = sP + g.fset_max;                              // dim 5: sound speed
     SlotRows r0;
     if (t < n_slots) r0 = load_slot(g, ext0 + uint32_t(t), edge_base);
     {
@@ -276,8 +280,13 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
         const float4* W4 = reinterpret_cast<const float4*>(W32);
         for (int q = t; q < n_fs; q += kBlock) {
             const uint32_t f = uint32_t(*row(g.fset_face, uint32_t(fs0 + q)));
-            sW[q] = *row(W4, f);
-            if constexpr (DIM == 5) sP[q] = *row(W32, uint32_t(4 * g.nf) + f);
+            const float4 w = *row(W4, f);
+            sW[q] = w;
+            if constexpr (DIM == 5) {
+                const float pq = *row(W32, uint32_t(4 * g.nf) + f);
+                sP[q] = pq;
+                sC[q] = m_sqrt(m_div(float(P.gam) * pq, w.x));
+            }
         }
     }
     __syncthreads();
@@ -289,8 +298,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
         const int t1 = t + kBlock;
         SlotRows r1;
         if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
-        if (t < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, r0, &sUP[t * DIM]);
-        if (t1 < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, r1, &sUP[t1 * DIM]);
+        if (t < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r0, &sUP[t * DIM]);
+        if (t1 < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r1, &sUP[t1 * DIM]);
     }
     uint32_t rec = 0;
     float4 e4, n4;
@@ -342,13 +351,11 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
         for (int k = 0; k < DIM; ++k) res[k] = 0;
         if constexpr (DIM == 5) {
             if (P.accretion_on) {
-                // the stage input (p-form) of this face, from LDS
-                const float4 x = sW[t];
-                const double w[5] = {x.x, x.y, x.z, x.w, sP[t]};
-                double src[5];
-                source_mixed(P, w, fcn, rate, src);
+                // on the face's stage input (p-form), from LDS
+                float src[5];
+                source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), float(rate), src);
 #pragma unroll
-                for (int k = 0; k < 5; ++k) res[k] = dt_here * src[k];
+                for (int k = 0; k < 5; ++k) res[k] = dt_here * double(src[k]);
             }
         }
         double sum[DIM], comp[DIM];
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
             }
         }
         if (nan_flux) raise_stop(sc, splash::kStopNaNFlux, i);
-        const double coef = P.round_diff * dt_here / area;
+        const double coef = rd_area * dt_here;
         double un[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) un[k] = (res[k] - coef * (sum[k] + comp[k])) + u0[k];
@@ -409,7 +416,7 @@ namespace {
 
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
-    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 20 : 16);
+    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 24 : 16);
     if (a.geom.fset_max > 1024 || lds + 2 * sizeof(float) * kSlots * DIM > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
