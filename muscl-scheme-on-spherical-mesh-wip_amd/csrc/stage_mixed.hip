@@ -33,7 +33,7 @@ __device__ __forceinline__ float m_tanh(float x) {
 // float4 (Sigma, L) row per face plus (dim 5) one pressure per face, staged
 // per block in LDS, and four float4 rows plus one packed 8-byte record per
 // slot.  The state, the RK combine, the dt limits and reductions and the
-// flux accumulation (compensated, Neumaier) stay FP64; the accretion /
+// flux accumulation (compensated, TwoSum) stay FP64; the accretion /
 // cooling / friction sources run in FP32 on the FP32 stage input.
 
 // Patches: at most kFaces faces (one wave updates them) whose own
@@ -172,8 +172,10 @@ __device__ __forceinline__ void source32(const PhysParams& P, float4 x, float p,
 
 // pressure_fc, find_v_max and extra_dt_constr terms of one E-form state
 // (as face_terms) with one FP64 reciprocal of the density instead of one
-// division per use: the p-form feeds the FP32 stage input, and the dt limits
-// stay FP64 within an ulp of the exact quotients.
+// division per use, and the friction dt 0.01 Sigma / (alpha rho |v - v0|)
+// with rho = gam3d/(2 gam3d - 1) g_eff Sigma^2 / p as one quotient: the
+// p-form feeds the FP32 stage input, and the dt limits stay FP64 within a
+// few ulp of the exact quotients.
 template <int DIM>
 __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const double* u, V3<double> fcn, double* w,
                                                  double& vmax, double& dtx) {
@@ -187,12 +189,10 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
         const double p = smax(1e-16, (u[4] - u[0] * vv2 * 0.5) * (P.gam - 1));
         w[4] = p;
         cs = sqrt(P.gam * p * inv);
-        const double gam3d = 1 / (2 - P.gam);
         const double g_eff = kGM - vv2;
-        const double rho = gam3d / (2 * gam3d - 1) * g_eff * u[0] * u[0] / p;
         const V3<double> vel = rl * (-inv);
         const V3<double> vel0 = cross(v3(0.0, 0.0, P.omega_ns), fcn);
-        const double cand = 0.01 * u[0] / (P.alpha * rho * norm(vel - vel0));
+        const double cand = 0.01 * p / (P.alpha * P.gam3d_rho * g_eff * u[0] * norm(vel - vel0));
         if (dtx > cand) dtx = cand;
     } else {
         cs = 1.0;
@@ -366,10 +366,13 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
             const float* Fh = &sF[(hh - hbase) * DIM];
 #pragma unroll
             for (int k = 0; k < DIM; ++k) {
+                // compensated sum: TwoSum's exact rounding error of each
+                // addition goes into comp
                 nan_flux |= __builtin_isnan(Fh[k]);
                 const double x = double(Fh[k]);
                 const double s = sum[k] + x;
-                comp[k] += (fabs(sum[k]) >= fabs(x)) ? (sum[k] - s) + x : (x - s) + sum[k];
+                const double xb = s - sum[k];
+                comp[k] += (sum[k] - (s - xb)) + (x - xb);
                 sum[k] = s;
             }
         }
