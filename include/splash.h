@@ -158,6 +158,39 @@ int splash_geometry_create(const splash_mesh_desc *mesh, int threads, splash_geo
 int splash_geometry_get_view(const splash_geometry *g, splash_geometry_view *out);
 void splash_geometry_free(splash_geometry *g);
 
+/* ---- Batched ensemble -------------------------------------------------
+ * No counterpart in the reference, which advances one run per solver
+ * object.  For callers that advance many independent runs at once
+ * (parameter sweeps, ensembles of initial states): n_runs runs of one
+ * solver and dim, each on its own mesh with its own state, gam, omega_ns
+ * and params[r], are advanced by the FP32-flux / FP64-state kernels with
+ * every RK stage one launch for all runs.  Each run keeps its own dt clamp,
+ * time and stop_check and evolves exactly as splash_create(...) with
+ * params.precision = SPLASH_MIXED would on the same inputs.
+ * params[r].precision is ignored; params[0].setup_threads sets the host
+ * threads of the geometry precompute.  U_in[r] is [n_faces(r)][dim]. */
+typedef struct splash_ensemble splash_ensemble;
+int splash_ensemble_create(int n_runs, const splash_mesh_desc *meshes, const double *const *U_in, int dim,
+                           int solver, const double *gam, const double *omega_ns, const splash_params *params,
+                           splash_ensemble **out);
+void splash_ensemble_destroy(splash_ensemble *e);
+/* nsteps do_step calls of every run, dt0[r] being run r's requested step;
+ * replayed from a captured HIP graph, returns without waiting. */
+int splash_ensemble_step(splash_ensemble *e, const double *dt0, int nsteps);
+/* Measurement extension, as splash_profile_step: n steps of every run with
+ * dispatch-bound HIP events; ms_out[4] = mean [dt finalize, stage 1,
+ * stage 2, whole step] of the ensemble's launches. */
+int splash_ensemble_profile_step(splash_ensemble *e, const double *dt0, int n, double *ms_out);
+int splash_ensemble_synchronize(splash_ensemble *e);
+/* n_runs, total faces, total half-edges, stage-kernel workgroups */
+int splash_ensemble_info(const splash_ensemble *e, int32_t *n_runs, int64_t *total_faces, int64_t *total_halfedges,
+                         int32_t *n_blocks);
+int splash_ensemble_num_faces(const splash_ensemble *e, int run, int32_t *n_faces);
+int splash_ensemble_time(splash_ensemble *e, int run, double *t);
+int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
+int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);
+int splash_ensemble_get_state(splash_ensemble *e, int run, double *U_out);
+
 /* Host-side check of the internal GPU layout of a mesh that the
  * mixed-precision kernel uses (Morton face order, patches of at most 64
  * faces whose half-edges plus halo fit 512 LDS slots, LDS face sets):

@@ -8,8 +8,10 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
+#include "ensemble.hpp"
 #include "geometry.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
@@ -23,6 +25,9 @@ struct splash_ctx {
 };
 struct splash_geometry {
     splash::Geometry geo;
+};
+struct splash_ensemble {
+    std::unique_ptr<splash::Ensemble> ens;
 };
 
 namespace {
@@ -51,6 +56,19 @@ int guarded(F&& f) {
     } catch (...) {
         return fail(SPLASH_EINTERNAL, "unknown error");
     }
+}
+
+splash::RunParams run_params(const splash_params& p) {
+    splash::RunParams rp;
+    rp.accretion_on = p.accretion_on;
+    rp.friction_on = p.friction_on;
+    rp.acc_rate = p.acc_rate;
+    rp.e_acc = p.e_acc;
+    rp.omega_acc_abs = p.omega_acc_abs;
+    rp.tilt_angle = p.tilt_angle;
+    rp.acc_width = p.acc_width;
+    rp.alpha = p.alpha;
+    return rp;
 }
 
 splash::PolyMesh from_desc(const splash_mesh_desc* d) {
@@ -131,15 +149,7 @@ int splash_create(const splash_mesh_desc* mesh, const double* U_in, int dim, int
         splash_params p;
         if (params) p = *params;
         else splash_default_params(&p);
-        splash::RunParams rp;
-        rp.accretion_on = p.accretion_on;
-        rp.friction_on = p.friction_on;
-        rp.acc_rate = p.acc_rate;
-        rp.e_acc = p.e_acc;
-        rp.omega_acc_abs = p.omega_acc_abs;
-        rp.tilt_angle = p.tilt_angle;
-        rp.acc_width = p.acc_width;
-        rp.alpha = p.alpha;
+        const splash::RunParams rp = run_params(p);
         splash::PolyMesh m = from_desc(mesh);
         auto ctx = std::make_unique<splash_ctx>();
         ctx->engine = std::make_unique<splash::Engine>(m, U_in, dim, solver, gam, omega_ns, rp, p.precision,
@@ -311,6 +321,119 @@ int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
         int32_t max_fset = 0;
         for (int32_t b = 0; b < L.n_blocks(); ++b) max_fset = std::max(max_fset, L.fset_off[b + 1] - L.fset_off[b]);
         out[4] = max_fset;
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_create(int n_runs, const splash_mesh_desc* meshes, const double* const* U_in, int dim,
+                           int solver, const double* gam, const double* omega_ns, const splash_params* params,
+                           splash_ensemble** out) {
+    if (!out) return fail(SPLASH_EINVAL, "out is null");
+    *out = nullptr;
+    if (n_runs <= 0 || !meshes || !U_in || !gam || !omega_ns || !params)
+        return fail(SPLASH_EINVAL, "bad ensemble arguments");
+    return guarded([&] {
+        std::vector<splash::PolyMesh> m;
+        m.reserve(size_t(n_runs));
+        std::vector<splash::Ensemble::Run> runs;
+        for (int r = 0; r < n_runs; ++r) {
+            m.push_back(from_desc(&meshes[r]));
+            runs.push_back({&m.back(), U_in[r], gam[r], omega_ns[r], run_params(params[r])});
+        }
+        auto e = std::make_unique<splash_ensemble>();
+        e->ens = std::make_unique<splash::Ensemble>(runs, dim, solver, params[0].setup_threads);
+        *out = e.release();
+        return SPLASH_OK;
+    });
+}
+
+void splash_ensemble_destroy(splash_ensemble* e) { delete e; }
+
+#define ENS_OR_FAIL                                                    \
+    if (!e || !e->ens) return fail(SPLASH_EINVAL, "null ensemble")
+#define RUN_OR_FAIL                                                    \
+    if (run < 0 || run >= e->ens->n_runs()) return fail(SPLASH_EINVAL, "run index out of range")
+
+int splash_ensemble_step(splash_ensemble* e, const double* dt0, int nsteps) {
+    ENS_OR_FAIL;
+    if (!dt0 || nsteps < 0) return fail(SPLASH_EINVAL, "bad step arguments");
+    return guarded([&] {
+        e->ens->steps(dt0, nsteps);
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_profile_step(splash_ensemble* e, const double* dt0, int n, double* ms_out) {
+    ENS_OR_FAIL;
+    if (!dt0 || n < 0 || !ms_out) return fail(SPLASH_EINVAL, "bad profile arguments");
+    return guarded([&] {
+        e->ens->profile(dt0, n, ms_out);
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_synchronize(splash_ensemble* e) {
+    ENS_OR_FAIL;
+    return guarded([&] {
+        e->ens->synchronize();
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_info(const splash_ensemble* e, int32_t* n_runs, int64_t* total_faces, int64_t* total_halfedges,
+                         int32_t* n_blocks) {
+    ENS_OR_FAIL;
+    if (n_runs) *n_runs = e->ens->n_runs();
+    if (total_faces) *total_faces = e->ens->total_faces();
+    if (total_halfedges) *total_halfedges = e->ens->total_halfedges();
+    if (n_blocks) *n_blocks = e->ens->n_blocks();
+    return SPLASH_OK;
+}
+
+int splash_ensemble_num_faces(const splash_ensemble* e, int run, int32_t* n_faces) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!n_faces) return fail(SPLASH_EINVAL, "null output");
+    *n_faces = e->ens->n_faces(run);
+    return SPLASH_OK;
+}
+
+int splash_ensemble_time(splash_ensemble* e, int run, double* t) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!t) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *t = e->ens->time(run);
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_last_dt(splash_ensemble* e, int run, double* dt) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!dt) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *dt = e->ens->last_dt(run);
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_stop_check(splash_ensemble* e, int run, int* stop) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!stop) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        *stop = e->ens->stop_bits(run) != 0;
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_get_state(splash_ensemble* e, int run, double* U_out) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!U_out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        e->ens->get_state(run, U_out);
         return SPLASH_OK;
     });
 }

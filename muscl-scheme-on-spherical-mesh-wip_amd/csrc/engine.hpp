@@ -14,19 +14,9 @@
 #include "geometry.hpp"
 #include "launch.hpp"
 #include "mesh.hpp"
+#include "run_setup.hpp"
 
 namespace splash {
-
-struct RunParams {
-    int32_t accretion_on = 0;
-    int32_t friction_on = 1;
-    double acc_rate = 3.25e-7;
-    double e_acc = 0.02;
-    double omega_acc_abs = 0.4;
-    double tilt_angle = 0.4;
-    double acc_width = 0.1;
-    double alpha = 1e-6;
-};
 
 class HipError : public std::runtime_error {
 public:
@@ -71,7 +61,7 @@ public:
 private:
     void enqueue_step(double dt0);
     void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr);
-    void upload();
+    void upload(RunHost& r);
     Scalars read_scalars();
 
     Geometry geo_;
@@ -81,10 +71,8 @@ private:
     double area_coeff_ = 0, total_mass_ = 0;
 
     // internal layout: faces along a Morton curve, half-edges face-major
-    std::vector<int32_t> perm_, iperm_;      // internal -> mesh face, mesh -> internal
-    std::vector<int32_t> foff_;              // internal face offsets
-    std::vector<int32_t> he_old_, he_new_;   // internal -> mesh half-edge and back
-    std::vector<int32_t> blk_face_, halo_off_, halo_he_, twin_slot_, fset_off_, fset_face_, ext_loc_;
+    std::vector<int32_t> perm_;              // internal -> mesh face
+    std::vector<int32_t> he_old_;            // internal -> mesh half-edge
 
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;

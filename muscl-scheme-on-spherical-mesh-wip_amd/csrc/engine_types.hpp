@@ -85,7 +85,11 @@ struct GeomPtrs {
     const double* rd_area;    // [nf] round_diff / area
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
-    double* dt_partial;       // [n_blocks][2]: max(|v|, c_s), min friction dt
+    double* dt_partial;       // [n_blocks][2] by logical block: max(|v|, c_s), min friction dt
+    // ensembles: several runs' patches in one grid (null / 1 for one run)
+    const int32_t* blk_run;   // [n_blocks] run of each logical block
+    const int32_t* run_blk;   // [n_runs+1] block range of each run
+    int32_t n_runs;
 };
 
 }  // namespace splash

@@ -1,0 +1,257 @@
+#include "ensemble.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace splash {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+void append(std::vector<T>& dst, const std::vector<T>& src) {
+    dst.insert(dst.end(), src.begin(), src.end());
+}
+
+}  // namespace
+
+void* Ensemble::dev(size_t bytes, const void* host) {
+    void* p = nullptr;
+    check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
+    allocations_.push_back(p);
+    if (host && bytes) check(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice), "upload");
+    return p;
+}
+
+Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads)
+    : dim_(dim), solver_(solver) {
+    if (runs.empty()) throw std::invalid_argument("an ensemble needs at least one run");
+    const int R = int(runs.size());
+
+    // merged tables: every run's arrays appended, indices shifted by the
+    // faces / half-edges / halo slots / face-set entries before it; only
+    // the trailing padding blocks of the whole grid are empty
+    std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
+    std::vector<uint32_t> slot_a, edge_rec;
+    std::vector<float> slot_b, edge32;
+    std::vector<double> fcn4, area, rate, rd_area, U;
+    std::vector<PhysParams> phys;
+    int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1;
+    for (int r = 0; r < R; ++r) {
+        const Run& run = runs[size_t(r)];
+        if (!run.mesh) throw std::invalid_argument("null mesh");
+        RunHost rh = build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
+                                    mixed::slots(), mixed::max_faces(), geometry_threads);
+        const MixedTables m = pack_mixed_tables(rh);
+        const Layout& L = rh.L;
+        const int32_t nf = rh.geo.nf, nh = rh.geo.nh;
+        int32_t nb = 0;  // the run's blocks without its own trailing padding
+        while (nb < L.n_blocks() && L.blk_face[nb + 1] > L.blk_face[nb]) ++nb;
+        if (int64_t(slot_a.size()) / 4 != int64_t(H) + halo) throw std::logic_error("ensemble row offsets");
+        run_blk.push_back(int32_t(blk_face.size()));
+        for (int32_t b = 0; b < nb; ++b) {
+            blk_face.push_back(L.blk_face[b] + F);
+            halo_off.push_back(L.halo_off[b] + halo);
+            fset_off.push_back(L.fset_off[b] + fsets);
+            blk_run.push_back(r);
+        }
+        for (int32_t k = 1; k <= nf; ++k) foff.push_back(L.foff[k] + H);
+        for (int32_t f : L.fset_face) fset_face.push_back(f + F);
+        append(slot_a, m.slot_a);
+        append(slot_b, m.slot_b);
+        append(edge32, m.edge32);
+        append(edge_rec, m.edge_rec);
+        append(rate, m.rate);
+        append(rd_area, m.rd_area);
+        append(fcn4, rh.fcn4);
+        append(area, rh.area);
+        for (int32_t k = 0; k < nf; ++k)
+            for (int c = 0; c < dim; ++c) U.push_back(run.U_in[size_t(L.perm[k]) * dim + c]);
+        phys.push_back(rh.phys);
+        fset_max = std::max(fset_max, m.fset_max);
+        faces_.push_back(nf);
+        face_off_.push_back(F);
+        perm_.push_back(L.perm);
+        F += nf;
+        H += nh;
+        halo += int32_t(L.halo_he.size());
+        fsets += int32_t(L.fset_face.size());
+    }
+    run_blk.push_back(int32_t(blk_face.size()));
+    blk_face.push_back(F);
+    halo_off.push_back(halo);
+    fset_off.push_back(fsets);
+    while ((blk_face.size() - 1) % 8) {  // the XCD-aware block numbering needs a multiple of 8
+        blk_face.push_back(F);
+        halo_off.push_back(halo);
+        fset_off.push_back(fsets);
+        blk_run.push_back(R - 1);
+    }
+    if (16 * (size_t(H) + size_t(halo)) >= (size_t(1) << 32) || 8 * size_t(F) * size_t(dim) >= (size_t(1) << 32))
+        throw std::runtime_error("ensemble too large for the mixed-precision path's 32-bit table offsets");
+
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    try {
+        GeomPtrs& d = args_.geom;
+        d = GeomPtrs{};
+        d.nf = F;
+        d.nh = H;
+        d.foff = static_cast<const int32_t*>(dev(sizeof(int32_t) * foff.size(), foff.data()));
+        d.area = static_cast<const double*>(dev(sizeof(double) * area.size(), area.data()));
+        d.fcn = static_cast<const double*>(dev(sizeof(double) * fcn4.size(), fcn4.data()));
+        d.blk_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_face.size(), blk_face.data()));
+        d.halo_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * halo_off.size(), halo_off.data()));
+        d.slot_a = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * slot_a.size(), slot_a.data()));
+        d.slot_b = static_cast<const float*>(dev(sizeof(float) * slot_b.size(), slot_b.data()));
+        d.edge32 = static_cast<const float*>(dev(sizeof(float) * edge32.size(), edge32.data()));
+        d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * edge_rec.size(), edge_rec.data()));
+        d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
+        d.rd_area = static_cast<const double*>(dev(sizeof(double) * rd_area.size(), rd_area.data()));
+        d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
+        d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));
+        d.fset_max = fset_max;
+        d.n_blocks = int32_t(blk_face.size()) - 1;
+        d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
+        d.blk_run = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_run.size(), blk_run.data()));
+        d.run_blk = static_cast<const int32_t*>(dev(sizeof(int32_t) * run_blk.size(), run_blk.data()));
+        d.n_runs = R;
+
+        args_.phys = phys[0];
+        args_.phys_dev = static_cast<const PhysParams*>(dev(sizeof(PhysParams) * phys.size(), phys.data()));
+        dt0_host_.assign(size_t(R), 0.0);
+        dt0_dev_ = static_cast<double*>(dev(sizeof(double) * size_t(R), dt0_host_.data()));
+        args_.dt0_dev = dt0_dev_;
+        std::vector<Scalars> sc(static_cast<size_t>(R));
+        for (auto& s : sc) {
+            s = Scalars{};
+            s.first_err = ~0ull;
+        }
+        args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars) * sc.size(), sc.data()));
+        args_.U = static_cast<double*>(dev(sizeof(double) * U.size(), U.data()));
+        args_.W = static_cast<double*>(dev(sizeof(double) * U.size(), nullptr));
+        args_.W2 = nullptr;
+        const size_t row = dim == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
+        args_.W32 = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
+        args_.W32b = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
+
+        fp64::enqueue_prepare(args_, stream_);
+        check(hipGetLastError(), "prepare launch");
+        check(hipStreamSynchronize(stream_), "prepare");
+    } catch (...) {
+        for (void* p : allocations_) (void)hipFree(p);
+        (void)hipStreamDestroy(stream_);
+        throw;
+    }
+}
+
+Ensemble::~Ensemble() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : allocations_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Ensemble::upload_dt0(const double* dt0) {
+    if (!dt0) throw std::invalid_argument("dt0 is null");
+    if (std::equal(dt0_host_.begin(), dt0_host_.end(), dt0)) return;
+    // the previous steps still read the old values: wait for them first
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    dt0_host_.assign(dt0, dt0 + dt0_host_.size());
+    check(hipMemcpy(dt0_dev_, dt0_host_.data(), sizeof(double) * dt0_host_.size(), hipMemcpyHostToDevice),
+          "upload dt0");
+}
+
+void Ensemble::enqueue_kernel(int which, const KernelEvents* ev) {
+    if (which == 0) fp64::enqueue_finalize_dt(args_, 0.0, stream_, ev);  // dt0 from the per-run array
+    else mixed::enqueue_stage(args_, which, stream_, ev);
+}
+
+void Ensemble::enqueue_step() {
+    for (int k = 0; k < kKernelsPerStep; ++k) enqueue_kernel(k);
+}
+
+void Ensemble::steps(const double* dt0, int n) {
+    upload_dt0(dt0);
+    if (n <= 0) return;
+    constexpr int kGraphSteps = 8;
+    if (n >= kGraphSteps) {
+        if (!graph_exec_) {  // dt0 lives in device memory, so one graph serves every call
+            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            for (int k = 0; k < kGraphSteps; ++k) enqueue_step();
+            check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+            check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+        }
+        while (n >= kGraphSteps) {
+            check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+            n -= kGraphSteps;
+        }
+    }
+    for (int k = 0; k < n; ++k) enqueue_step();
+    check(hipGetLastError(), "steps launch");
+}
+
+void Ensemble::profile(const double* dt0, int n, double* ms) {
+    upload_dt0(dt0);
+    constexpr int K = kKernelsPerStep;
+    const size_t nk = size_t(n > 0 ? n : 0) * K;
+    std::vector<hipEvent_t> ev(2 * nk, nullptr);
+    double acc[K + 1] = {0};
+    try {
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        for (size_t q = 0; q < nk; ++q) {
+            const KernelEvents ke{ev[2 * q], ev[2 * q + 1]};
+            enqueue_kernel(int(q % K), &ke);
+        }
+        check(hipGetLastError(), "profile launch");
+        if (nk) check(hipEventSynchronize(ev.back()), "hipEventSynchronize");
+        for (size_t q = 0; q < nk; ++q) {
+            float t = 0;
+            check(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]), "hipEventElapsedTime");
+            acc[q % K] += t;
+            if (q % K == K - 1) {
+                check(hipEventElapsedTime(&t, ev[2 * (q - K + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
+                acc[K] += t;
+            }
+        }
+    } catch (...) {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k <= K; ++k) ms[k] = n > 0 ? acc[k] / n : 0.0;
+}
+
+void Ensemble::synchronize() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+Scalars Ensemble::read_scalars(int r) {
+    if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    Scalars s;
+    check(hipMemcpyAsync(&s, args_.sc + r, sizeof(Scalars), hipMemcpyDeviceToHost, stream_), "read scalars");
+    synchronize();
+    return s;
+}
+
+double Ensemble::time(int r) { return read_scalars(r).t; }
+double Ensemble::last_dt(int r) { return read_scalars(r).dt; }
+int Ensemble::stop_bits(int r) { return read_scalars(r).stop; }
+
+void Ensemble::get_state(int r, double* U_out) {
+    if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    const int32_t nf = faces_[size_t(r)];
+    std::vector<double> U(size_t(nf) * dim_);
+    check(hipMemcpyAsync(U.data(), args_.U + size_t(face_off_[size_t(r)]) * dim_, U.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, stream_),
+          "read state");
+    synchronize();
+    const std::vector<int32_t>& perm = perm_[size_t(r)];
+    for (int32_t k = 0; k < nf; ++k)
+        for (int c = 0; c < dim_; ++c) U_out[size_t(perm[k]) * dim_ + c] = U[size_t(k) * dim_ + c];
+}
+
+}  // namespace splash

@@ -1,0 +1,69 @@
+// Several independent SPLASH runs (uneven meshes, their own parameters and
+// dt0) advanced together on one GPU.  Their patches are concatenated into
+// one grid, so each RK stage is a single launch of the FP32-flux /
+// FP64-state kernel for every run; the dt of each run is reduced over its
+// own patches by one workgroup per run.  Each run evolves exactly as a
+// single mixed-precision Engine on the same inputs would.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "engine.hpp"
+#include "run_setup.hpp"
+
+namespace splash {
+
+class Ensemble {
+public:
+    struct Run {
+        const PolyMesh* mesh;
+        const double* U_in;  // [nf][dim] in the mesh's face order
+        double gam, omega_ns;
+        RunParams params;
+    };
+    Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads = 0);
+    ~Ensemble();
+    Ensemble(const Ensemble&) = delete;
+    Ensemble& operator=(const Ensemble&) = delete;
+
+    void steps(const double* dt0, int n);  // dt0: [n_runs]; n steps, graph-replayed in groups of 8
+    void synchronize();
+    static constexpr int kKernelsPerStep = 3;
+    // n back-to-back steps, each kernel timed with dispatch-bound HIP events:
+    // ms = mean [dt finalize, stage 1, stage 2, whole step]
+    void profile(const double* dt0, int n, double* ms);
+
+    int n_runs() const { return int(faces_.size()); }
+    int32_t n_faces(int r) const { return faces_.at(size_t(r)); }
+    int64_t total_faces() const { return args_.geom.nf; }
+    int64_t total_halfedges() const { return args_.geom.nh; }
+    int32_t n_blocks() const { return args_.geom.n_blocks; }
+    double time(int r);
+    double last_dt(int r);
+    int stop_bits(int r);
+    void get_state(int r, double* U_out);
+
+private:
+    void upload_dt0(const double* dt0);
+    void enqueue_step();
+    void enqueue_kernel(int which, const KernelEvents* ev = nullptr);
+    Scalars read_scalars(int r);
+    void* dev(size_t bytes, const void* host);
+
+    int dim_, solver_;
+    std::vector<int32_t> faces_, face_off_;       // per run
+    std::vector<std::vector<int32_t>> perm_;      // per run: internal -> mesh face
+    std::vector<double> dt0_host_;
+
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    StepArgs args_{};
+    double* dt0_dev_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+};
+
+}  // namespace splash

@@ -10,8 +10,10 @@ namespace splash {
 
 struct StepArgs {
     GeomPtrs geom;
-    PhysParams phys;
-    Scalars* sc;
+    PhysParams phys;                // one run's parameters (FP64 stage kernel argument)
+    const PhysParams* phys_dev;     // [n_runs] on the device
+    const double* dt0_dev;          // [n_runs] per-run dt0 (ensembles), or null: the dt0 argument
+    Scalars* sc;                    // [n_runs]
     double* U;  // [nf][dim] E-form state
     double* W;   // [nf][dim] p-form input of stage 1 (and output of stage 2)
     double* W2;  // [nf][dim] p-form input of stage 2

@@ -1,0 +1,68 @@
+// Host-side set-up of one run: geometry, the internal GPU layout, the
+// per-half-edge and per-face tables in the layout's order, the physical
+// parameters of the kernels, and (mixed precision) the packed FP32 tables.
+// One resident run (Engine) and an ensemble of runs (Ensemble) build their
+// device data from it.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "engine_types.hpp"
+#include "geometry.hpp"
+#include "layout.hpp"
+#include "mesh.hpp"
+
+namespace splash {
+
+struct RunParams {
+    int32_t accretion_on = 0;
+    int32_t friction_on = 1;
+    double acc_rate = 3.25e-7;
+    double e_acc = 0.02;
+    double omega_acc_abs = 0.4;
+    double tilt_angle = 0.4;
+    double acc_width = 0.1;
+    double alpha = 1e-6;
+};
+
+struct RunHost {
+    Geometry geo;
+    Layout L;
+    int dim = 0, solver = 0;
+    double area_coeff = 0, total_mass = 0;
+    // internal half-edge order
+    std::vector<int32_t> he_fv, sten;             // owner | valence << 27; stencil faces
+    std::vector<double> beta, hsc, ec, en;         // [nh][4] each
+    // internal face order
+    std::vector<double> area, fcn4, acc;           // [nf], [nf][4], [nf][8]
+    PhysParams phys{};
+};
+
+// Checks the arguments (throws std::invalid_argument), builds the geometry
+// with `threads` OpenMP threads and the layout with the given patch shape.
+RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
+                       const RunParams& params, int32_t slots, int32_t max_faces, int threads);
+
+// FP32 tables of the mixed kernel, per extended slot row (block b's own then
+// halo half-edges start at foff[f0] + halo_off[b]) and per edge of a block
+// (a block holds both slots of each of its edges, so n_slots/2 edges from
+// row (foff[f0] + halo_off[b]) / 2), the edge oriented for the slot that
+// evaluates its flux:
+//   slot_a [n_ext][4]: x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30,
+//                      y = minus1 | owner << 10 | edge << 20 | valence << 28
+//                      (face-set locals), then beta+0, beta-0 as float bits
+//                      (beta.1 = 1 - beta.0 to 3e-13 on every generated mesh)
+//   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
+//   edge32 [n_ext/2][8]: unit edge midpoint, |ec|, normal, chord
+//   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower-index << 18
+//   rate, rd_area [nf]: accretion rate profile, round_diff / area
+struct MixedTables {
+    std::vector<uint32_t> slot_a, edge_rec;
+    std::vector<float> slot_b, edge32;
+    std::vector<double> rate, rd_area;
+    int32_t fset_max = 1;
+};
+MixedTables pack_mixed_tables(const RunHost& r);
+
+}  // namespace splash

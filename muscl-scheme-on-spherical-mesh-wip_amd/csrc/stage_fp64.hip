@@ -35,13 +35,15 @@ void enqueue_stage(const StepArgs& a, int st, hipStream_t s, const KernelEvents*
 }
 
 void enqueue_prepare(const StepArgs& a, hipStream_t s) {
-    if (a.phys.dim == 5) k_prepare<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W, a.W32);
-    else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys, a.U, a.W, a.W32);
+    if (a.phys.dim == 5) k_prepare<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W, a.W32);
+    else k_prepare<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W, a.W32);
 }
 
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev) {
-    if (a.phys.dim == 5) launch(k_finalize_dt<5>, 1, kFinalizeThreads, 0, s, ev, a.geom, a.phys, a.sc, dt0);
-    else launch(k_finalize_dt<4>, 1, kFinalizeThreads, 0, s, ev, a.geom, a.phys, a.sc, dt0);
+    const int runs = a.geom.n_runs;
+    if (a.phys.dim == 5)
+        launch(k_finalize_dt<5>, runs, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0, a.dt0_dev);
+    else launch(k_finalize_dt<4>, runs, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0, a.dt0_dev);
 }
 
 

@@ -234,12 +234,16 @@ __device__ __forceinline__ void reconstruct_slot(const PhysParams& P, float dt, 
 }
 
 template <int DIM, int SOLVER, int STAGE>
-__global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P, Scalars* __restrict__ sc,
-                                                    double* __restrict__ U, const float* __restrict__ W32,
-                                                    float* __restrict__ W32_out) {
+__global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysParams* __restrict__ Pv,
+                                                       Scalars* __restrict__ scv, double* __restrict__ U,
+                                                       const float* __restrict__ W32,
+                                                       float* __restrict__ W32_out) {
     __shared__ float sUP[kSlots * DIM];
     __shared__ float sF[kSlots * DIM];  // edge flux times chord length, per own slot
     const int b = xcd_block();
+    const int run = g.blk_run ? g.blk_run[b] : 0;  // ensembles: this patch's run
+    const PhysParams P = Pv[run];
+    Scalars* __restrict__ sc = scv + run;
     const int f0 = g.blk_face[b], f1 = g.blk_face[b + 1];
     const int hbase = g.foff[f0];
     const int n_own = g.foff[f1] - hbase;
@@ -404,9 +408,9 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, PhysParams P,
                 if (dtx > ox) dtx = ox;
             }
             if (t == 0) {
-                g.dt_partial[2 * blockIdx.x] = vmax;
-                g.dt_partial[2 * blockIdx.x + 1] = dtx;
-                if (blockIdx.x == 0) {
+                g.dt_partial[2 * b] = vmax;
+                g.dt_partial[2 * b + 1] = dtx;
+                if (b == (g.run_blk ? g.run_blk[run] : 0)) {  // the run's first patch advances its clock
                     sc->t += sc->dt;
                     sc->steps += 1;
                 }
@@ -423,11 +427,11 @@ void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) 
     if (a.geom.fset_max > 1024 || lds + 2 * sizeof(float) * kSlots * DIM > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
-        launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys, a.sc, a.U, a.W32,
-               a.W32b);
+        launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
+               a.W32, a.W32b);
     else
-        launch(k_stage32<DIM, SOLVER, 2>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys, a.sc, a.U, a.W32b,
-               a.W32);
+        launch(k_stage32<DIM, SOLVER, 2>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
+               a.W32b, a.W32);
 }
 
 }  // namespace

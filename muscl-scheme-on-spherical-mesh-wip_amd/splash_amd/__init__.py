@@ -15,7 +15,7 @@ from ._lib import (FP64, HLLC, HLLCPLUS, HLLE, HLLE_P, MIXED, GeometryView, Mesh
                    SplashError, check, lib)
 
 __all__ = ["SurfaceMesh", "uv_sphere", "icosphere", "icosphere_hex", "quad_sphere", "Geometry",
-           "MUSCL_HLLE", "MUSCL_HLLE_p", "MUSCL_HLLC", "MUSCL_HLLCplus", "default_params",
+           "MUSCL_HLLE", "MUSCL_HLLE_p", "MUSCL_HLLC", "MUSCL_HLLCplus", "Ensemble", "default_params",
            "SplashError", "FP64", "MIXED", "ic"]
 
 
@@ -240,3 +240,81 @@ class MUSCL_HLLC(_MUSCL):
 class MUSCL_HLLCplus(_MUSCL):
     """MUSCL_HLLCplus(mesh, U_in, dim, gam, omega_ns, accretion_on, threads) -- HLLCplus.hpp:11."""
     _solver = HLLCPLUS
+
+
+class Ensemble:
+    """Independent runs of one solver advanced together (splash_ensemble_*).
+
+    runs: list of (mesh, U_in, gam, omega_ns, params); every RK stage is one
+    launch of the FP32-flux / FP64-state kernels for all runs, and each run
+    evolves as MUSCL_*(..., precision=MIXED) would on the same inputs."""
+
+    def __init__(self, runs, dim, solver):
+        n = len(runs)
+        self._descs = (MeshDesc * n)(*[m.desc() for m, *_ in runs])
+        self._meshes = [m for m, *_ in runs]
+        self._U = [np.ascontiguousarray(U, dtype=np.float64) for _, U, *_ in runs]
+        for (m, *_), U in zip(runs, self._U):
+            if U.shape != (m.n_faces, dim):
+                raise ValueError(f"U_in must be [{m.n_faces}][{dim}], got {U.shape}")
+        ptrs = (C.POINTER(C.c_double) * n)(*[_dp(U) for U in self._U])
+        gam = np.array([r[2] for r in runs], dtype=np.float64)
+        om = np.array([r[3] for r in runs], dtype=np.float64)
+        params = (Params * n)(*[r[4] if r[4] is not None else default_params() for r in runs])
+        self._dim = dim
+        self._e = C.c_void_p()
+        check(lib().splash_ensemble_create(n, self._descs, ptrs, dim, solver, _dp(gam), _dp(om), params,
+                                           C.byref(self._e)))
+        self.n_runs = n
+
+    def __del__(self):
+        e = getattr(self, "_e", None)
+        if e:
+            lib().splash_ensemble_destroy(e)
+            self._e = None
+
+    def _dt0(self, dt0):
+        a = np.broadcast_to(np.asarray(dt0, dtype=np.float64), (self.n_runs,))
+        return np.ascontiguousarray(a)
+
+    def do_steps(self, dt0, n):
+        """n steps of every run; dt0 is a scalar or one value per run."""
+        d = self._dt0(dt0)
+        check(lib().splash_ensemble_step(self._e, _dp(d), int(n)))
+
+    def profile_step(self, dt0, n):
+        d = self._dt0(dt0)
+        out = np.zeros(4)
+        check(lib().splash_ensemble_profile_step(self._e, _dp(d), int(n), _dp(out)))
+        return out
+
+    def synchronize(self):
+        check(lib().splash_ensemble_synchronize(self._e))
+
+    def info(self):
+        """(n_runs, total faces, total half-edges, stage-kernel workgroups)."""
+        r, f, h, b = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().splash_ensemble_info(self._e, C.byref(r), C.byref(f), C.byref(h), C.byref(b)))
+        return r.value, f.value, h.value, b.value
+
+    def time(self, run):
+        t = C.c_double()
+        check(lib().splash_ensemble_time(self._e, int(run), C.byref(t)))
+        return t.value
+
+    def last_dt(self, run):
+        t = C.c_double()
+        check(lib().splash_ensemble_last_dt(self._e, int(run), C.byref(t)))
+        return t.value
+
+    def get_stop_check(self, run):
+        s = C.c_int()
+        check(lib().splash_ensemble_stop_check(self._e, int(run), C.byref(s)))
+        return bool(s.value)
+
+    def get_state(self, run):
+        n = C.c_int32()
+        check(lib().splash_ensemble_num_faces(self._e, int(run), C.byref(n)))
+        out = np.empty((n.value, self._dim))
+        check(lib().splash_ensemble_get_state(self._e, int(run), _dp(out)))
+        return out

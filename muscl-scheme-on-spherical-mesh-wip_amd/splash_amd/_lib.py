@@ -74,6 +74,21 @@ SIGNATURES = {
     "splash_geometry_get_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
     "splash_geometry_free": (None, [C.c_void_p]),
     "splash_layout_stats": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_int32)]),
+    "splash_ensemble_create": (C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(C.POINTER(C.c_double)), C.c_int,
+                                         C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),
+                                         C.POINTER(C.c_void_p)]),
+    "splash_ensemble_destroy": (None, [C.c_void_p]),
+    "splash_ensemble_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
+    "splash_ensemble_profile_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                                               C.POINTER(C.c_double)]),
+    "splash_ensemble_synchronize": (C.c_int, [C.c_void_p]),
+    "splash_ensemble_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "splash_ensemble_num_faces": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32)]),
+    "splash_ensemble_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "splash_ensemble_last_dt": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "splash_ensemble_stop_check": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "splash_ensemble_get_state": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_last_error_message": (C.c_char_p, []),
 }
 

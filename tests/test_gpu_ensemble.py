@@ -1,0 +1,74 @@
+"""Batched ensemble (splash_ensemble_*) against single runs.
+
+Every run of an ensemble is advanced by the same kernels as a single
+MIXED-precision solver on the same inputs, with its own dt clamp, time and
+stop_check; the bar is bit-for-bit equality with the single runs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GAM = 2 - 1 / (4 / 3)
+
+
+def _runs(S):
+    acc = S.default_params(accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1, omega_acc_abs=0.8,
+                           tilt_angle=0.3, acc_width=0.1, alpha=1e-6)
+    spec = [(S.icosphere_hex(3), 0.0, None, (0, 0, 0.3), 1e-4),
+            (S.icosphere_hex(4), 0.1, acc, (0.2, 0, 0.1), 2e-4),
+            (S.quad_sphere(4), 0.0, None, (0, 0.3, 0), 1e-4),
+            (S.uv_sphere(32, 16), 0.2, acc, (0.1, 0.1, 0.1), 5e-5),
+            (S.icosphere(3), 0.0, None, (0, 0, 0.5), 1e-4)]
+    out = []
+    for k, (m, om, p, omega, dt0) in enumerate(spec):
+        U0 = S.ic.adiabatic_state(m.face_centres(), GAM - 0.01 * k, seed=10 + k, omega=omega)
+        out.append((m, U0, GAM - 0.01 * k, om, p, dt0))
+    return out
+
+
+def test_ensemble_runs_equal_single_mixed_runs():
+    import splash_amd as S
+    runs = _runs(S)
+    ens = S.Ensemble([(m, U0, g, om, p) for m, U0, g, om, p, _ in runs], 5, S.HLLCPLUS)
+    n_runs, faces, halfedges, blocks = ens.info()
+    assert n_runs == len(runs) and faces == sum(m.n_faces for m, *_ in runs) and blocks % 8 == 0
+    dt0 = [r[5] for r in runs]
+    singles = []
+    for m, U0, g, om, p, d in runs:
+        s = S.MUSCL_HLLCplus(m, U0, 5, g, om, p is not None, params=p, precision=S.MIXED)
+        singles.append(s)
+    for n in (19, 8, 3):  # graph replays plus eager remainders
+        ens.do_steps(dt0, n)
+        for s, d in zip(singles, dt0):
+            s.do_steps(d, n)
+    for r, (s, d) in enumerate(zip(singles, dt0)):
+        A, B = s.get_state(), ens.get_state(r)
+        print(f"run {r}: max |diff| {np.max(np.abs(A - B)):.3e}, t {ens.time(r):.6g}")
+        assert np.array_equal(A, B)
+        assert ens.time(r) == s.time()
+        assert ens.last_dt(r) == s.last_dt()
+        assert ens.get_stop_check(r) == s.get_stop_check()
+
+
+def test_ensemble_isothermal_and_changing_dt0():
+    """dim 4 / MUSCL_HLLE runs, and a per-run dt0 that changes between calls."""
+    import splash_amd as S
+    meshes = [S.icosphere_hex(3), S.quad_sphere(3)]
+    U = [S.ic.isothermal_state(m.face_centres(), seed=3 + k, sigma_amp=0.05) for k, m in enumerate(meshes)]
+    ens = S.Ensemble([(m, u, GAM, 0.0, None) for m, u in zip(meshes, U)], 4, S.HLLE)
+    singles = [S.MUSCL_HLLE(m, u, 4, GAM, precision=S.MIXED) for m, u in zip(meshes, U)]
+    for dt0 in ([1e-3, 5e-4], [2e-4, 2e-4]):
+        ens.do_steps(dt0, 12)
+        for s, d in zip(singles, dt0):
+            s.do_steps(d, 12)
+    for r, s in enumerate(singles):
+        assert np.array_equal(s.get_state(), ens.get_state(r))
+        assert ens.time(r) == s.time()
+
+
+def test_ensemble_rejects_mixed_solver_shapes():
+    import splash_amd as S
+    m = S.icosphere_hex(2)
+    U4 = S.ic.isothermal_state(m.face_centres(), seed=1)
+    with pytest.raises(S.SplashError):
+        S.Ensemble([(m, U4, GAM, 0.0, None)], 4, S.HLLCPLUS)  # dim 4 with an adiabatic solver
