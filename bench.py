@@ -105,6 +105,80 @@ def profile(s, dt0, n):
     return s.profile_step(dt0, n)
 
 
+def ensemble_runs(S, n_runs=64, seed=4):
+    """C4: n_runs icosphere_hex meshes with levels drawn from {4, 5, 6, 7}
+    (seed 4), run i seeded 4000 + i: Sigma = 1 + 0.1 N, Pi = 0.1 Sigma,
+    solid-body rotation about a uniform random axis with Omega ~ U(0, 1)
+    plus 0.01 N tangential noise."""
+    levels = 4 + np.minimum((S.ic.uniform(seed, 0, n_runs) * 4).astype(int), 3)
+    meshes = {lv: S.icosphere_hex(int(lv)) for lv in sorted(set(levels.tolist()))}
+    runs = []
+    for i, lv in enumerate(levels.tolist()):
+        si = 4000 + i
+        axis = S.ic.gaussian(si, 7, 3)
+        axis /= np.linalg.norm(axis)
+        omega = S.ic.uniform(si, 8, 1)[0] * axis
+        m = meshes[lv]
+        U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=si, sigma_amp=0.1, pi_frac=0.1,
+                                  omega=tuple(omega), noise=0.01)
+        runs.append((m, U0, lv))
+    return runs
+
+
+def ensemble_bench(S, steps, warmup, dt0, profile_steps):
+    """The C4 ensemble: one launch per stage for all 64 runs, against the same
+    runs advanced one after another by single solvers (FP32-flux and FP64)."""
+    runs = ensemble_runs(S)
+    p = params(S, False)
+    ens = S.Ensemble([(m, U0, GAM, 0.0, p) for m, U0, _ in runs], 5, S.HLLCPLUS)
+    n_runs, F, H, blocks = ens.info()
+    ens.do_steps(dt0, warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.do_steps(dt0, steps)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    Ue = [ens.get_state(r) for r in range(n_runs)]
+    stops = sum(ens.get_stop_check(r) for r in range(n_runs))
+    ens.do_steps(dt0, 64)
+    prof = ens.profile_step(dt0, profile_steps)
+    del ens
+    # the same runs one at a time
+    seq = {}
+    num = den = 0.0
+    linf = 0.0
+    for prec, name in ((S.MIXED, "mixed"), (S.FP64, "fp64")):
+        tot = 0.0
+        for r, (m, U0, _) in enumerate(runs):
+            s = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, False, params=p, precision=prec)
+            s.do_steps(dt0, warmup)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.do_steps(dt0, steps)
+            s.synchronize()
+            tot += time.perf_counter() - t0
+            if prec == S.FP64:
+                A = s.get_state()
+                num += float(np.sum(np.abs(A[:, 0] - Ue[r][:, 0])))
+                den += float(np.sum(np.abs(A[:, 0])))
+                linf = max(linf, float(np.max(np.abs(A[:, 0] - Ue[r][:, 0])) / np.max(np.abs(A[:, 0]))))
+            del s
+        seq[name] = F * steps / tot
+    ab = algorithmic_bytes("stage_2", F, H)
+    return {
+        "workload": f"c4: {n_runs} icosphere_hex runs, levels {sorted(set(lv for *_, lv in runs))} drawn with "
+                    f"seed 4, F={F} H={H}, dim 5 MUSCL_HLLCplus, per-run seeds 4000+i, random axis, "
+                    f"Omega~U(0,1), dt0={dt0}, {steps} steps",
+        "value": F * steps / el, "unit": "face-updates/s", "ms_per_step": el / steps * 1e3,
+        "one_run_at_a_time": {"mixed": seq["mixed"], "fp64": seq["fp64"]},
+        "speedup_vs_fp64_runs": (F * steps / el) / seq["fp64"],
+        "l1_err_vs_fp64": num / den, "linf_err_vs_fp64": linf,
+        "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
+        "stage_2_hbm_frac": ab / (prof[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "workgroups_per_stage": blocks, "runs_with_stop_check": int(stops),
+    }
+
+
 def cpu_baseline(S, wl, U0, dt0, steps, threads):
     import oracle_lib as O
     kind, margs, solver, omega_ns, acc, ic, _ = wl
@@ -130,6 +204,8 @@ def main():
     ap.add_argument("--cpu-face-updates", type=float, default=3e7,
                     help="size of the CPU baseline sample in face-updates (whole steps, at least one)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble (C4) measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
@@ -220,6 +296,8 @@ def main():
         "stop_check": bool(stop),
         "sim_time": [t64, tmx],
     }
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_bench(S, args.ensemble_steps, args.warmup, args.dt, args.profile_steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_steps = max(1, int(round(args.cpu_face_updates / F)))
