@@ -71,6 +71,8 @@ typedef struct splash_params {
     double alpha;
     int32_t precision; /* SPLASH_FP64 or SPLASH_MIXED */
     int32_t setup_threads; /* host threads for the geometry precompute, 0 = 1 */
+    int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..64; 0 = 64;
+                          * -1 = the fastest of a few candidates, measured at creation */
 } splash_params;
 
 typedef struct splash_mesh splash_mesh;   /* library-owned generated mesh */
@@ -129,6 +131,9 @@ int splash_get_state(splash_ctx *ctx, double *U_out);
  * out = {sum Sigma*A, sum Lx*A, sum Ly*A, sum Lz*A, sum E*A}. */
 int splash_diagnostics(splash_ctx *ctx, double out[5]);
 int splash_num_faces(const splash_ctx *ctx, int32_t *n_faces, int32_t *n_halfedges);
+/* faces per stage-kernel patch in use (SPLASH_MIXED; the tuned value when
+ * params.patch_faces = -1) */
+int splash_patch_faces(const splash_ctx *ctx, int32_t *patch_faces);
 /* Face centres (MUSCL_geometry.hpp:176-187) and areas (:989-1030). */
 int splash_face_geometry(const splash_ctx *ctx, double *centers_out, double *areas_out);
 
@@ -186,6 +191,8 @@ int splash_ensemble_synchronize(splash_ensemble *e);
 int splash_ensemble_info(const splash_ensemble *e, int32_t *n_runs, int64_t *total_faces, int64_t *total_halfedges,
                          int32_t *n_blocks);
 int splash_ensemble_num_faces(const splash_ensemble *e, int run, int32_t *n_faces);
+/* faces per patch in use (the tuned value when params[0].patch_faces = -1) */
+int splash_ensemble_patch_faces(const splash_ensemble *e, int32_t *patch_faces);
 int splash_ensemble_time(splash_ensemble *e, int run, double *t);
 int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
 int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);

@@ -114,6 +114,7 @@ void splash_default_params(splash_params* p) {
     p->alpha = d.alpha;
     p->precision = SPLASH_FP64;
     p->setup_threads = 0;
+    p->patch_faces = 0;
 }
 
 int splash_mesh_uv_sphere(int n_slices, int n_stacks, splash_mesh** out) {
@@ -153,7 +154,7 @@ int splash_create(const splash_mesh_desc* mesh, const double* U_in, int dim, int
         splash::PolyMesh m = from_desc(mesh);
         auto ctx = std::make_unique<splash_ctx>();
         ctx->engine = std::make_unique<splash::Engine>(m, U_in, dim, solver, gam, omega_ns, rp, p.precision,
-                                                       p.setup_threads);
+                                                       p.setup_threads, p.patch_faces);
         *out = ctx.release();
         return SPLASH_OK;
     });
@@ -259,6 +260,13 @@ int splash_num_faces(const splash_ctx* ctx, int32_t* n_faces, int32_t* n_halfedg
     return SPLASH_OK;
 }
 
+int splash_patch_faces(const splash_ctx* ctx, int32_t* patch_faces) {
+    CTX_OR_FAIL;
+    if (!patch_faces) return fail(SPLASH_EINVAL, "null output");
+    *patch_faces = ctx->engine->precision() == SPLASH_MIXED ? ctx->engine->patch_faces() : 0;
+    return SPLASH_OK;
+}
+
 int splash_face_geometry(const splash_ctx* ctx, double* centers_out, double* areas_out) {
     CTX_OR_FAIL;
     const splash::Geometry& g = ctx->engine->geometry();
@@ -341,7 +349,8 @@ int splash_ensemble_create(int n_runs, const splash_mesh_desc* meshes, const dou
             runs.push_back({&m.back(), U_in[r], gam[r], omega_ns[r], run_params(params[r])});
         }
         auto e = std::make_unique<splash_ensemble>();
-        e->ens = std::make_unique<splash::Ensemble>(runs, dim, solver, params[0].setup_threads);
+        e->ens = std::make_unique<splash::Ensemble>(runs, dim, solver, params[0].setup_threads,
+                                                    params[0].patch_faces);
         *out = e.release();
         return SPLASH_OK;
     });
@@ -395,6 +404,13 @@ int splash_ensemble_num_faces(const splash_ensemble* e, int run, int32_t* n_face
     RUN_OR_FAIL;
     if (!n_faces) return fail(SPLASH_EINVAL, "null output");
     *n_faces = e->ens->n_faces(run);
+    return SPLASH_OK;
+}
+
+int splash_ensemble_patch_faces(const splash_ensemble* e, int32_t* patch_faces) {
+    ENS_OR_FAIL;
+    if (!patch_faces) return fail(SPLASH_EINVAL, "null output");
+    *patch_faces = e->ens->patch_faces();
     return SPLASH_OK;
 }
 

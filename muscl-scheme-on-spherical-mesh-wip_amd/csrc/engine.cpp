@@ -5,6 +5,7 @@
 #include "run_setup.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -21,34 +22,73 @@ void check(hipError_t e, const char* what) {
 }  // namespace
 
 Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-               const RunParams& params, int precision, int geometry_threads)
+               const RunParams& params, int precision, int geometry_threads, int patch_faces)
     : dim_(dim), solver_(solver), precision_(precision), gam_(gam), omega_ns_(omega_ns), params_(params) {
     if (precision != kFP64 && precision != kMixed) throw std::invalid_argument("unknown precision mode");
+    if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
+    const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
     RunHost r = precision == kMixed
-                    ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(),
-                                     mixed::max_faces(), geometry_threads)
+                    ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(), cap,
+                                     geometry_threads)
                     : build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, fp64::block_size(),
                                      fp64::block_size(), geometry_threads);
+    if (precision == kMixed && patch_faces == -1) autotune_patch_faces(r, U_in, 1e-4);
+    init(r, U_in);
+    geo_ = std::move(r.geo);
+}
+
+Engine::Engine(const RunHost& r, const double* U_in, int precision)
+    : dim_(r.dim), solver_(r.solver), precision_(precision), gam_(r.phys.gam), omega_ns_(r.phys.omega_ns),
+      params_(r.params) {
+    if (precision != kFP64 && precision != kMixed) throw std::invalid_argument("unknown precision mode");
+    init(r, U_in);
+}
+
+void Engine::init(const RunHost& r, const double* U_in) {
     area_coeff_ = r.area_coeff;
     total_mass_ = r.total_mass;
+    patch_faces_ = r.L.max_faces;
+    nf_ = r.geo.nf;
+    nh_ = r.geo.nh;
     const int32_t nf = r.geo.nf;
-
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     try {
         upload(r);
-        std::vector<double> U(size_t(nf) * dim);
+        std::vector<double> U(size_t(nf) * dim_);
         for (int32_t k = 0; k < nf; ++k)
-            for (int c = 0; c < dim; ++c) U[size_t(k) * dim + c] = U_in[size_t(perm_[k]) * dim + c];
+            for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm_[k]) * dim_ + c];
         check(hipMemcpy(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice), "upload state");
         fp64::enqueue_prepare(args_, stream_);
         check(hipGetLastError(), "prepare launch");
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {
         for (void* p : allocations_) (void)hipFree(p);
+        allocations_.clear();
         (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
         throw;
     }
-    geo_ = std::move(r.geo);
+}
+
+int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0) {
+    int32_t best = kPatchFacesMax;
+    double best_t = 0;
+    for (int32_t cap : kPatchFaceCandidates) {
+        relayout(r, mixed::slots(), cap);
+        Engine e(r, U_in, kMixed);
+        e.steps(dt0, 16);
+        e.synchronize();
+        const auto t0 = std::chrono::steady_clock::now();
+        e.steps(dt0, 32);
+        e.synchronize();
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (best_t == 0 || t < best_t) {
+            best_t = t;
+            best = cap;
+        }
+    }
+    if (r.L.max_faces != best) relayout(r, mixed::slots(), best);
+    return best;
 }
 
 Engine::~Engine() {
@@ -59,10 +99,10 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void Engine::upload(RunHost& r) {
+void Engine::upload(const RunHost& r) {
     const Geometry& g = r.geo;
     const int32_t nf = g.nf, nh = g.nh;
-    Layout& L = r.L;
+    const Layout& L = r.L;
     auto dev = [&](size_t bytes, const void* host) -> void* {
         void* p = nullptr;
         check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
@@ -133,8 +173,8 @@ void Engine::upload(RunHost& r) {
     n_diag_blocks_ = (nf + B - 1) / B;
     d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
 
-    perm_ = std::move(L.perm);
-    he_old_ = std::move(L.he_old);
+    perm_ = L.perm;
+    he_old_ = L.he_old;
 }
 
 void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev) {
@@ -233,10 +273,10 @@ uint64_t Engine::first_error() {
 }
 
 void Engine::get_state(double* U_out) {
-    std::vector<double> U(size_t(geo_.nf) * dim_);
+    std::vector<double> U(size_t(nf_) * dim_);
     check(hipMemcpyAsync(U.data(), args_.U, sizeof(double) * U.size(), hipMemcpyDeviceToHost, stream_), "read state");
     synchronize();
-    for (int32_t k = 0; k < geo_.nf; ++k)
+    for (int32_t k = 0; k < nf_; ++k)
         for (int c = 0; c < dim_; ++c) U_out[size_t(perm_[k]) * dim_ + c] = U[size_t(k) * dim_ + c];
 }
 

@@ -26,8 +26,12 @@ public:
 class Engine {
 public:
     // U_in is [nf][dim] in mesh face order (slot 4 = energy for dim 5).
+    // patch_faces (mixed precision): faces per patch of the stage kernel,
+    // 1..64, 0 = 64, -1 = the fastest of a few measured candidates
     Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-           const RunParams& params, int precision, int geometry_threads = 0);
+           const RunParams& params, int precision, int geometry_threads = 0, int patch_faces = 0);
+    // from a prepared run, whose layout sets the patch shape
+    Engine(const RunHost& r, const double* U_in, int precision);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -48,12 +52,14 @@ public:
     void get_state(double* U_out);      // [nf][dim], slot 4 = energy
     void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
 
-    int32_t n_faces() const { return geo_.nf; }
-    int32_t n_halfedges() const { return geo_.nh; }
+    int32_t n_faces() const { return nf_; }
+    int32_t n_halfedges() const { return nh_; }
     int dim() const { return dim_; }
     int precision() const { return precision_; }
     double area_coeff() const { return area_coeff_; }
     double total_mass() const { return total_mass_; }
+    int32_t patch_faces() const { return patch_faces_; }
+    // the host geometry tables (empty for an Engine built from a RunHost)
     const Geometry& geometry() const { return geo_; }
     hipStream_t stream() const { return stream_; }
     const StepArgs& args() const { return args_; }
@@ -61,7 +67,8 @@ public:
 private:
     void enqueue_step(double dt0);
     void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr);
-    void upload(RunHost& r);
+    void init(const RunHost& r, const double* U_in);
+    void upload(const RunHost& r);
     Scalars read_scalars();
 
     Geometry geo_;
@@ -69,6 +76,8 @@ private:
     double gam_, omega_ns_;
     RunParams params_;
     double area_coeff_ = 0, total_mass_ = 0;
+    int32_t patch_faces_ = 0;
+    int32_t nf_ = 0, nh_ = 0;
 
     // internal layout: faces along a Morton curve, half-edges face-major
     std::vector<int32_t> perm_;              // internal -> mesh face
@@ -85,5 +94,15 @@ private:
     double graph_dt0_ = 0;
     int graph_steps_ = 0;
 };
+
+// Mixed-precision patch face cap with the shortest measured step: each
+// candidate re-lays out r, runs a trial Engine on its own copy of the state
+// (16 graph-replayed warm-up steps, then 32 timed ones at dt0) and r is left
+// laid out for the winner, which is returned.
+int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0);
+
+// faces per patch the mixed kernel allows, and the candidates of the tuner
+constexpr int32_t kPatchFacesMax = 64;
+constexpr int32_t kPatchFaceCandidates[] = {64, 48, 40, 32, 24};
 
 }  // namespace splash

@@ -1,6 +1,7 @@
 #include "ensemble.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <stdexcept>
 #include <string>
 
@@ -27,10 +28,54 @@ void* Ensemble::dev(size_t bytes, const void* host) {
     return p;
 }
 
-Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads)
+Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads, int patch_faces)
     : dim_(dim), solver_(solver) {
     if (runs.empty()) throw std::invalid_argument("an ensemble needs at least one run");
-    const int R = int(runs.size());
+    if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
+    const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
+    std::vector<RunHost> hosts;
+    std::vector<const double*> U_in;
+    for (const Run& run : runs) {
+        if (!run.mesh) throw std::invalid_argument("null mesh");
+        hosts.push_back(build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
+                                       mixed::slots(), cap, geometry_threads));
+        U_in.push_back(run.U_in);
+    }
+    if (patch_faces == -1) {
+        // the fastest patch cap for the whole ensemble, measured on trial
+        // ensembles with their own copies of the states
+        int32_t best = kPatchFacesMax;
+        double best_t = 0;
+        std::vector<double> dt0(runs.size(), 1e-4);
+        for (int32_t c : kPatchFaceCandidates) {
+            for (RunHost& h : hosts) relayout(h, mixed::slots(), c);
+            Ensemble trial(hosts, U_in, dim, solver);
+            trial.steps(dt0.data(), 16);
+            trial.synchronize();
+            const auto t0 = std::chrono::steady_clock::now();
+            trial.steps(dt0.data(), 32);
+            trial.synchronize();
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (best_t == 0 || t < best_t) {
+                best_t = t;
+                best = c;
+            }
+        }
+        for (RunHost& h : hosts)
+            if (h.L.max_faces != best) relayout(h, mixed::slots(), best);
+    }
+    init(hosts, U_in);
+}
+
+Ensemble::Ensemble(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in, int dim, int solver)
+    : dim_(dim), solver_(solver) {
+    init(hosts, U_in);
+}
+
+void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in) {
+    const int dim = dim_;
+    const int R = int(hosts.size());
+    patch_faces_ = hosts[0].L.max_faces;
 
     // merged tables: every run's arrays appended, indices shifted by the
     // faces / half-edges / halo slots / face-set entries before it; only
@@ -42,10 +87,8 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
     std::vector<PhysParams> phys;
     int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1;
     for (int r = 0; r < R; ++r) {
-        const Run& run = runs[size_t(r)];
-        if (!run.mesh) throw std::invalid_argument("null mesh");
-        RunHost rh = build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
-                                    mixed::slots(), mixed::max_faces(), geometry_threads);
+        const RunHost& rh = hosts[size_t(r)];
+        if (rh.dim != dim || rh.solver != solver_) throw std::invalid_argument("ensemble runs differ in dim or solver");
         const MixedTables m = pack_mixed_tables(rh);
         const Layout& L = rh.L;
         const int32_t nf = rh.geo.nf, nh = rh.geo.nh;
@@ -70,7 +113,7 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
         append(fcn4, rh.fcn4);
         append(area, rh.area);
         for (int32_t k = 0; k < nf; ++k)
-            for (int c = 0; c < dim; ++c) U.push_back(run.U_in[size_t(L.perm[k]) * dim + c]);
+            for (int c = 0; c < dim; ++c) U.push_back(U_in[size_t(r)][size_t(L.perm[k]) * dim + c]);
         phys.push_back(rh.phys);
         fset_max = std::max(fset_max, m.fset_max);
         faces_.push_back(nf);

@@ -24,7 +24,9 @@ public:
         double gam, omega_ns;
         RunParams params;
     };
-    Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads = 0);
+    // patch_faces as for Engine: 1..64, 0 = 64, -1 = the fastest of a few
+    // candidates measured on trial ensembles
+    Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads = 0, int patch_faces = 0);
     ~Ensemble();
     Ensemble(const Ensemble&) = delete;
     Ensemble& operator=(const Ensemble&) = delete;
@@ -41,12 +43,15 @@ public:
     int64_t total_faces() const { return args_.geom.nf; }
     int64_t total_halfedges() const { return args_.geom.nh; }
     int32_t n_blocks() const { return args_.geom.n_blocks; }
+    int32_t patch_faces() const { return patch_faces_; }
     double time(int r);
     double last_dt(int r);
     int stop_bits(int r);
     void get_state(int r, double* U_out);
 
 private:
+    Ensemble(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in, int dim, int solver);
+    void init(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in);
     void upload_dt0(const double* dt0);
     void enqueue_step();
     void enqueue_kernel(int which, const KernelEvents* ev = nullptr);
@@ -54,6 +59,7 @@ private:
     void* dev(size_t bytes, const void* host);
 
     int dim_, solver_;
+    int32_t patch_faces_ = 0;
     std::vector<int32_t> faces_, face_off_;       // per run
     std::vector<std::vector<int32_t>> perm_;      // per run: internal -> mesh face
     std::vector<double> dt0_host_;

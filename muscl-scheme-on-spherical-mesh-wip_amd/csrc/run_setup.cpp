@@ -30,7 +30,7 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
     r.solver = solver;
     r.geo = build_geometry(mesh, threads);
     const Geometry& g = r.geo;
-    const int32_t nf = g.nf, nh = g.nh;
+    const int32_t nf = g.nf;
     if (g.max_valence >= 32) throw std::runtime_error("face valence >= 32 is not supported");
     if (dim == 5) {
         // adiabatic.hpp:70-90 (the index of the :75 loop starts at face 0)
@@ -50,6 +50,36 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
         r.total_mass = tm;
     }
 
+    PhysParams& P = r.phys;
+    P.dim = dim;
+    P.solver = solver;
+    P.accretion_on = params.accretion_on;
+    P.friction_on = params.friction_on;
+    P.gam = gam;
+    P.omega_ns = omega_ns;
+    P.h0 = g.h0;
+    P.round_diff = g.round_diff;
+    P.e_acc = params.e_acc;
+    P.alpha = params.alpha;
+    const double fall_eff = 1;
+    P.dmdt_coef = dim == 5 ? -(fall_eff * params.acc_rate * 4 * M_PI) / r.total_mass : 0.0;
+    P.om_acc[0] = 0;
+    P.om_acc[1] = std::sin(params.tilt_angle) * params.omega_acc_abs;
+    P.om_acc[2] = std::cos(params.tilt_angle) * params.omega_acc_abs;
+    const double gam3d = 1 / (2 - gam);
+    P.gam_h = gam / (gam - 1);
+    P.gam3d_rho = gam3d / (2 * gam3d - 1);
+    P.cool_k = 12. / 5 * 1.6e-13 * std::pow(3. / 4 * 4.85e36, 0.25);  // adiabatic.hpp cooling constants
+    r.params = params;
+    relayout(r, slots, max_faces);
+    return r;
+}
+
+void relayout(RunHost& r, int32_t slots, int32_t max_faces) {
+    const Geometry& g = r.geo;
+    const int32_t nf = g.nf, nh = g.nh;
+    const RunParams& params = r.params;
+    const int dim = r.dim;
     r.L = build_layout(g, slots, max_faces);
     validate_layout(g, r.L);
     const Layout& L = r.L;
@@ -118,27 +148,6 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
         }
     }
 
-    PhysParams& P = r.phys;
-    P.dim = dim;
-    P.solver = solver;
-    P.accretion_on = params.accretion_on;
-    P.friction_on = params.friction_on;
-    P.gam = gam;
-    P.omega_ns = omega_ns;
-    P.h0 = g.h0;
-    P.round_diff = g.round_diff;
-    P.e_acc = params.e_acc;
-    P.alpha = params.alpha;
-    const double fall_eff = 1;
-    P.dmdt_coef = dim == 5 ? -(fall_eff * params.acc_rate * 4 * M_PI) / r.total_mass : 0.0;
-    P.om_acc[0] = 0;
-    P.om_acc[1] = std::sin(params.tilt_angle) * params.omega_acc_abs;
-    P.om_acc[2] = std::cos(params.tilt_angle) * params.omega_acc_abs;
-    const double gam3d = 1 / (2 - gam);
-    P.gam_h = gam / (gam - 1);
-    P.gam3d_rho = gam3d / (2 * gam3d - 1);
-    P.cool_k = 12. / 5 * 1.6e-13 * std::pow(3. / 4 * 4.85e36, 0.25);  // adiabatic.hpp cooling constants
-    return r;
 }
 
 MixedTables pack_mixed_tables(const RunHost& r) {

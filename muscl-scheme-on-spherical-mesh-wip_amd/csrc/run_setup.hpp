@@ -37,12 +37,17 @@ struct RunHost {
     // internal face order
     std::vector<double> area, fcn4, acc;           // [nf], [nf][4], [nf][8]
     PhysParams phys{};
+    RunParams params;
 };
 
 // Checks the arguments (throws std::invalid_argument), builds the geometry
 // with `threads` OpenMP threads and the layout with the given patch shape.
 RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
                        const RunParams& params, int32_t slots, int32_t max_faces, int threads);
+
+// Rebuilds the layout and every table in its order for another patch shape
+// (the geometry and physical parameters are kept).
+void relayout(RunHost& r, int32_t slots, int32_t max_faces);
 
 // FP32 tables of the mixed kernel, per extended slot row (block b's own then
 // halo half-edges start at foff[f0] + halo_off[b]) and per edge of a block

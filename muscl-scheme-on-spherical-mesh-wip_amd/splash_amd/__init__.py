@@ -218,6 +218,12 @@ class _MUSCL:
         check(lib().splash_diagnostics(self._ctx, _dp(out)))
         return out
 
+    def patch_faces(self):
+        """Faces per stage-kernel patch (mixed precision; 0 for FP64)."""
+        n = C.c_int32()
+        check(lib().splash_patch_faces(self._ctx, C.byref(n)))
+        return n.value
+
 
 class MUSCL_HLLE(_MUSCL):
     """MUSCL_HLLE(mesh, U_in, dim, gam, threads) -- HLLE.hpp:13, isothermal dim 4."""
@@ -296,6 +302,11 @@ class Ensemble:
         r, f, h, b = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().splash_ensemble_info(self._e, C.byref(r), C.byref(f), C.byref(h), C.byref(b)))
         return r.value, f.value, h.value, b.value
+
+    def patch_faces(self):
+        n = C.c_int32()
+        check(lib().splash_ensemble_patch_faces(self._e, C.byref(n)))
+        return n.value
 
     def time(self, run):
         t = C.c_double()

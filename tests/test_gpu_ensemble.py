@@ -72,3 +72,39 @@ def test_ensemble_rejects_mixed_solver_shapes():
     U4 = S.ic.isothermal_state(m.face_centres(), seed=1)
     with pytest.raises(S.SplashError):
         S.Ensemble([(m, U4, GAM, 0.0, None)], 4, S.HLLCPLUS)  # dim 4 with an adiabatic solver
+
+
+def test_patch_shape_and_autotuner_leave_results_unchanged():
+    """Each edge's flux is evaluated from its lower-index half-edge with the
+    same inputs whatever patch holds it, so the state does not depend on the
+    patch face cap: fixed caps and the tuned one agree bit for bit."""
+    import splash_amd as S
+    m = S.icosphere_hex(5)
+    U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=4, omega=(0, 0, 0.3))
+    out = []
+    for cap in (64, 32, 17, -1):
+        p = S.default_params(patch_faces=cap)
+        s = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=p, precision=S.MIXED)
+        got = s.patch_faces()
+        assert (got == cap) if cap > 0 else got in (64, 48, 40, 32, 24)
+        s.do_steps(1e-4, 40)
+        out.append(s.get_state())
+        print(f"patch_faces {cap} -> {got}")
+    for U in out[1:]:
+        assert np.array_equal(U, out[0])
+
+
+def test_ensemble_autotuned_equals_single_runs():
+    import splash_amd as S
+    runs = _runs(S)[:3]
+    tuned = S.default_params(patch_faces=-1)
+    ens = S.Ensemble([(m, U0, g, om, p if p is not None else tuned) for m, U0, g, om, p, _ in runs], 5,
+                     S.HLLCPLUS)
+    print("ensemble patch faces:", ens.patch_faces())
+    assert ens.patch_faces() in (64, 48, 40, 32, 24)
+    dt0 = [r[5] for r in runs]
+    ens.do_steps(dt0, 20)
+    for r, (m, U0, g, om, p, d) in enumerate(runs):
+        s = S.MUSCL_HLLCplus(m, U0, 5, g, om, p is not None, params=p, precision=S.MIXED)
+        s.do_steps(d, 20)
+        assert np.array_equal(s.get_state(), ens.get_state(r))
