@@ -130,6 +130,11 @@ int splash_get_state(splash_ctx *ctx, double *U_out);
 /* The do_step diagnostics sums (MUSCL_base.hpp:187-205):
  * out = {sum Sigma*A, sum Lx*A, sum Ly*A, sum Lz*A, sum E*A}. */
 int splash_diagnostics(splash_ctx *ctx, double out[5]);
+/* get_light_curves (adiabatic.hpp:228-333), adiabatic solvers only: the
+ * pressure-weighted projected flux seen by observers along x, (x+z)/sqrt2,
+ * z and -z, out = {flux_0, flux_45, flux_90, flux_180}; one device
+ * reduction, no state transfer. */
+int splash_light_curves(splash_ctx *ctx, double out[4]);
 int splash_num_faces(const splash_ctx *ctx, int32_t *n_faces, int32_t *n_halfedges);
 /* faces per stage-kernel patch in use (SPLASH_MIXED; the tuned value when
  * params.patch_faces = -1) */

@@ -317,27 +317,11 @@ public:
         }
     }
     // adiabatic.hpp:228-333: pressure-weighted projected flux towards four observers
+    // adiabatic.hpp:228-333, evaluated by one device reduction
     std::vector<double> get_light_curves() {
-        auto U = get_state();
-        const double d = 9.4 * 3 * 1e19 / 15000, s = std::sqrt(9.4 * 3 * 1e19 / 15000);
-        const double obs[4][3] = {{d, 0, 0}, {s, 0, s}, {0, 0, d}, {0, 0, -d}};
-        double tot[4] = {0, 0, 0, 0};
-        for (size_t i = 0; i < U.size(); ++i) {
-            double r[3];
-            unit_centre(i, r);
-            const double phi = std::atan2(r[1], r[0]);
-            const double theta = std::acos(fc_[3 * i + 2] / norm3(&fc_[3 * i]));
-            const double PI = pressure_at_centre(U[i], i);
-            auto cos_to = [&](int k) {
-                const double dv = obs[k][0] * r[0] + obs[k][1] * r[1] + obs[k][2] * r[2];
-                return std::abs(dv) / norm3(obs[k]);
-            };
-            if (phi < M_PI / 2 && phi > -M_PI / 2) tot[0] += PI * cos_to(0) * area_[i];
-            if (theta < M_PI / 2) tot[2] += PI * cos_to(2) * area_[i];
-            if (obs[1][0] * r[0] + obs[1][1] * r[1] + obs[1][2] * r[2] > 0) tot[1] += PI * cos_to(1) * area_[i];
-            if (theta > M_PI / 2) tot[3] += PI * cos_to(3) * area_[i];
-        }
-        return {tot[0], tot[1], tot[2], tot[3]};
+        std::vector<double> out(4);
+        check_rc(splash_light_curves(ctx_, out.data()));
+        return out;
     }
 
 protected:

@@ -260,6 +260,15 @@ int splash_num_faces(const splash_ctx* ctx, int32_t* n_faces, int32_t* n_halfedg
     return SPLASH_OK;
 }
 
+int splash_light_curves(splash_ctx* ctx, double out[4]) {
+    CTX_OR_FAIL;
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        ctx->engine->light_curves(out);
+        return SPLASH_OK;
+    });
+}
+
 int splash_patch_faces(const splash_ctx* ctx, int32_t* patch_faces) {
     CTX_OR_FAIL;
     if (!patch_faces) return fail(SPLASH_EINVAL, "null output");

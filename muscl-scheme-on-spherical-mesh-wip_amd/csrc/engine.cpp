@@ -172,6 +172,10 @@ void Engine::upload(const RunHost& r) {
     const int B = fp64::block_size();
     n_diag_blocks_ = (nf + B - 1) / B;
     d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
+    if (dim_ == 5) {
+        d_light_w_ = static_cast<double*>(dev(sizeof(double) * r.light_w.size(), r.light_w.data()));
+        d_light_part_ = static_cast<double*>(dev(sizeof(double) * 4 * n_diag_blocks_, nullptr));
+    }
 
     perm_ = L.perm;
     he_old_ = L.he_old;
@@ -290,6 +294,20 @@ void Engine::diagnostics(double out[5]) {
     for (int k = 0; k < 5; ++k) out[k] = 0;
     for (int b = 0; b < n_diag_blocks_; ++b)
         for (int k = 0; k < 5; ++k) out[k] += part[5 * size_t(b) + k];
+}
+
+void Engine::light_curves(double out[4]) {
+    if (dim_ != 5) throw std::invalid_argument("light curves exist for the adiabatic (dim 5) solvers only");
+    fp64::enqueue_light(args_, d_light_w_, d_light_part_, stream_);
+    check(hipGetLastError(), "light curve launch");
+    std::vector<double> part(4 * size_t(n_diag_blocks_));
+    check(hipMemcpyAsync(part.data(), d_light_part_, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                         stream_),
+          "read light curves");
+    synchronize();
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    for (int b = 0; b < n_diag_blocks_; ++b)
+        for (int k = 0; k < 4; ++k) out[k] += part[4 * size_t(b) + k];
 }
 
 }  // namespace splash

@@ -51,6 +51,7 @@ public:
     uint64_t first_error();
     void get_state(double* U_out);      // [nf][dim], slot 4 = energy
     void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
+    void light_curves(double out[4]);   // get_light_curves (dim 5)
 
     int32_t n_faces() const { return nf_; }
     int32_t n_halfedges() const { return nh_; }
@@ -87,6 +88,8 @@ private:
     std::vector<void*> allocations_;
     StepArgs args_{};
     double* d_partial_ = nullptr;
+    double* d_light_w_ = nullptr;       // [nf][4] observer weights (dim 5)
+    double* d_light_part_ = nullptr;    // [diag blocks][4]
     int n_diag_blocks_ = 0;
 
     hipGraphExec_t graph_exec_ = nullptr;

@@ -32,6 +32,8 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev = nullptr);
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
 void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s);
+// per-block partial sums [ceil(nf/block_size())][4] of the light curves (dim 5)
+void enqueue_light(const StepArgs& a, const double* weights, double* partial, hipStream_t s);
 int block_size();
 }  // namespace fp64
 

@@ -122,6 +122,34 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces) {
         for (int c = 0; c < 3; ++c) r.fcn4[4 * size_t(k) + c] = g.fcn[3 * size_t(f) + c];
     }
 
+    // get_light_curves (adiabatic.hpp:228-333): each observer's flux is
+    // sum_f Pi_f * cos(alpha) * A_f over the faces it sees, so the geometric
+    // factor mask * |r.o| / |o| * A is precomputed per face and observer
+    // (observers at 9400 pc along x, (x+z), z and -z)
+    r.light_w.assign(dim == 5 ? 4 * size_t(nf) : 0, 0.0);
+    if (dim == 5) {
+        const double d = 9.4 * 3 * 1e19 / 15000, sq = std::sqrt(9.4 * 3 * 1e19 / 15000);
+        const double obs[4][3] = {{d, 0, 0}, {sq, 0, sq}, {0, 0, d}, {0, 0, -d}};
+        for (int32_t k = 0; k < nf; ++k) {
+            const int32_t f = L.perm[k];
+            const double* c = &g.fc[3 * size_t(f)];
+            const double n = norm3(c[0], c[1], c[2]);
+            const double* u = &r.fcn4[4 * size_t(k)];
+            const double phi = std::atan2(c[1] / n, c[0] / n);
+            const double theta = std::acos(c[2] / n);
+            auto cos_a = [&](int o) {
+                const double dv = obs[o][0] * u[0] + obs[o][1] * u[1] + obs[o][2] * u[2];
+                return std::abs(dv) / norm3(obs[o][0], obs[o][1], obs[o][2]);
+            };
+            const double A = r.area[k];
+            double* w = &r.light_w[4 * size_t(k)];
+            if (phi < M_PI / 2 && phi > -M_PI / 2) w[0] = cos_a(0) * A;
+            if (obs[1][0] * u[0] + obs[1][1] * u[1] + obs[1][2] * u[2] > 0) w[1] = cos_a(1) * A;
+            if (theta < M_PI / 2) w[2] = cos_a(2) * A;
+            if (theta > M_PI / 2) w[3] = cos_a(3) * A;
+        }
+    }
+
     // accretion profile per face (adiabatic.hpp:425-449), evaluated once with
     // the expressions the reference evaluates on every source() call
     r.acc.assign(8 * size_t(nf), 0.0);

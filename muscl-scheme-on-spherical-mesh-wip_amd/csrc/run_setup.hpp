@@ -36,6 +36,7 @@ struct RunHost {
     std::vector<double> beta, hsc, ec, en;         // [nh][4] each
     // internal face order
     std::vector<double> area, fcn4, acc;           // [nf], [nf][4], [nf][8]
+    std::vector<double> light_w;                   // [nf][4] (dim 5): observer weights of get_light_curves
     PhysParams phys{};
     RunParams params;
 };

@@ -53,6 +53,11 @@ void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s) {
     else k_diag_partial<4><<<gf, kBlock, 0, s>>>(a.geom, a.U, partial);
 }
 
+void enqueue_light(const StepArgs& a, const double* weights, double* partial, hipStream_t s) {
+    const int gf = (a.geom.nf + kBlock - 1) / kBlock;
+    k_light_partial<<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, weights, partial);
+}
+
 int block_size() { return kBlock; }
 
 }  // namespace fp64

@@ -218,6 +218,12 @@ class _MUSCL:
         check(lib().splash_diagnostics(self._ctx, _dp(out)))
         return out
 
+    def get_light_curves(self):
+        """adiabatic.hpp:228-333: [flux_0, flux_45, flux_90, flux_180]."""
+        out = np.empty(4)
+        check(lib().splash_light_curves(self._ctx, _dp(out)))
+        return out
+
     def patch_faces(self):
         """Faces per stage-kernel patch (mixed precision; 0 for FP64)."""
         n = C.c_int32()

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from lightcurve_ref import light_curves
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "drop_in_driver")
@@ -62,3 +63,9 @@ def test_driver_loop_matches_oracle(tmp_path):
     assert rho.shape == (1 + PARAMS["maxstep"] // PARAMS["skipstep"], 1 + mesh.n_faces)
     fc = np.loadtxt(tmp_path / "results" / "face_centers.dat")
     assert fc.shape == (mesh.n_faces, 3) and np.allclose(np.linalg.norm(fc, axis=1), 1, atol=1e-14)
+    # the light curve written every step (get_light_curves on the device)
+    lc = np.loadtxt(tmp_path / "results" / "lightcurve0.dat")
+    assert lc.shape == (1 + PARAMS["maxstep"], 2)
+    g = S.Geometry(mesh)
+    want = light_curves(ref.state(), g.fc, g.area, GAM)[0]
+    assert lc[-1, 0] == pytest.approx(ref.time(), rel=1e-5) and lc[-1, 1] == pytest.approx(want, rel=1e-5)

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from lightcurve_ref import light_curves
 
 pytestmark = pytest.mark.gpu
 
@@ -218,3 +219,18 @@ def test_mixed_precision_every_solver_and_mesh(solver, kind, args, sources):
     assert mx.time() == pytest.approx(f64.time(), rel=1e-12)
     if not sources:
         assert abs(mx.diagnostics()[0] / d0[0] - 1) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_light_curves_device_reduction(precision):
+    import splash_amd as S
+    mesh = S.icosphere_hex(5)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=6, omega=(0.1, 0, 0.3))
+    s = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=getattr(S, precision))
+    s.do_steps(1e-4, 30)
+    g = S.Geometry(mesh)
+    want = light_curves(s.get_state(), g.fc, g.area, GAM)
+    got = s.get_light_curves()
+    print(f"light curves {precision}: {got}")
+    assert np.all(got > 0)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
