@@ -127,6 +127,11 @@ int splash_stop_check(splash_ctx *ctx, int *stop);
 int splash_stop_bits(splash_ctx *ctx, int *bits, uint64_t *first);
 /* Extension (the reference keeps U private): copies the state out. */
 int splash_get_state(splash_ctx *ctx, double *U_out);
+/* Resume (no counterpart in the reference, which can only restart from a
+ * text input file): replaces the state, the time and the stop_check bits
+ * (splash_stop_bits' codes) of a context, as read back from a checkpoint;
+ * U_in as for splash_create. */
+int splash_set_state(splash_ctx *ctx, const double *U_in, double t, int stop_bits);
 /* The do_step diagnostics sums (MUSCL_base.hpp:187-205):
  * out = {sum Sigma*A, sum Lx*A, sum Ly*A, sum Lz*A, sum E*A}. */
 int splash_diagnostics(splash_ctx *ctx, double out[5]);

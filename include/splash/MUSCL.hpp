@@ -16,7 +16,8 @@
 // per-step diagnostics line (set_verbose(false) silences it).
 //
 // `threads` (the OpenMP team size of the reference) is accepted and ignored.
-// Extensions: get_state(), do_steps(dt, n), set_verbose(), precision modes.
+// Extensions: get_state(), do_steps(dt, n), set_verbose(), precision modes,
+// write_checkpoint() / read_checkpoint().
 #pragma once
 
 #include <cmath>
@@ -32,6 +33,8 @@
 #include "../splash.h"
 
 namespace splash {
+
+constexpr char kCheckpointTag[9] = "SPLASHC1";
 
 inline void check_rc(int rc) {
     if (rc != SPLASH_OK) throw std::runtime_error(std::string("splash: ") + splash_last_error_message());
@@ -172,6 +175,43 @@ public:
         for (size_t i = 0; i < U.size(); ++i)
             for (int k = 0; k < dim_; ++k) U[i][size_t(k)] = flat[i * size_t(dim_) + size_t(k)];
         return U;
+    }
+
+    // Lossless binary checkpoint, in place of restarting from the text of
+    // write_final_state: an 8-byte tag, n_faces and dim (int64), the time, the
+    // stop_check bits (int64) and the state as raw FP64 in mesh face order.
+    void write_checkpoint(const std::string& path) {
+        std::vector<double> flat(mesh_.n_faces() * size_t(dim_));
+        check_rc(splash_get_state(ctx_, flat.data()));
+        int bits = 0;
+        uint64_t first = 0;
+        check_rc(splash_stop_bits(ctx_, &bits, &first));
+        const double t = time();
+        std::ofstream out(path, std::ios::binary);
+        const int64_t head[4] = {int64_t(mesh_.n_faces()), int64_t(dim_), int64_t(bits), 0};
+        out.write(kCheckpointTag, 8);
+        out.write(reinterpret_cast<const char*>(head), sizeof(head));
+        out.write(reinterpret_cast<const char*>(&t), sizeof(t));
+        out.write(reinterpret_cast<const char*>(flat.data()), std::streamsize(flat.size() * sizeof(double)));
+        if (!out) throw std::runtime_error("cannot write checkpoint " + path);
+    }
+    // Restores a checkpoint of the same mesh and dim (throws otherwise).
+    void read_checkpoint(const std::string& path) {
+        std::ifstream in(path, std::ios::binary);
+        char tag[8];
+        int64_t head[4];
+        double t = 0;
+        in.read(tag, 8);
+        in.read(reinterpret_cast<char*>(head), sizeof(head));
+        in.read(reinterpret_cast<char*>(&t), sizeof(t));
+        if (!in || std::string(tag, 8) != std::string(kCheckpointTag, 8))
+            throw std::runtime_error("not a checkpoint: " + path);
+        if (head[0] != int64_t(mesh_.n_faces()) || head[1] != int64_t(dim_))
+            throw std::runtime_error("checkpoint of another mesh or dim: " + path);
+        std::vector<double> flat(mesh_.n_faces() * size_t(dim_));
+        in.read(reinterpret_cast<char*>(flat.data()), std::streamsize(flat.size() * sizeof(double)));
+        if (!in) throw std::runtime_error("truncated checkpoint: " + path);
+        check_rc(splash_set_state(ctx_, flat.data(), t, int(head[2])));
     }
 
     // MUSCL_geometry.hpp:1069-1117

@@ -244,6 +244,15 @@ int splash_get_state(splash_ctx* ctx, double* U_out) {
     });
 }
 
+int splash_set_state(splash_ctx* ctx, const double* U_in, double t, int stop_bits) {
+    CTX_OR_FAIL;
+    if (!U_in) return fail(SPLASH_EINVAL, "null state");
+    return guarded([&] {
+        ctx->engine->set_state(U_in, t, stop_bits);
+        return SPLASH_OK;
+    });
+}
+
 int splash_diagnostics(splash_ctx* ctx, double out[5]) {
     CTX_OR_FAIL;
     if (!out) return fail(SPLASH_EINVAL, "null output");

@@ -58,8 +58,7 @@ void Engine::init(const RunHost& r, const double* U_in) {
         for (int32_t k = 0; k < nf; ++k)
             for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm_[k]) * dim_ + c];
         check(hipMemcpy(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice), "upload state");
-        fp64::enqueue_prepare(args_, stream_);
-        check(hipGetLastError(), "prepare launch");
+        enqueue_prepare();
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {
         for (void* p : allocations_) (void)hipFree(p);
@@ -162,8 +161,11 @@ void Engine::upload(const RunHost& r) {
     args_.phys_dev = static_cast<const PhysParams*>(dev(sizeof(PhysParams), &r.phys));
     args_.dt0_dev = nullptr;
     args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
-    args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
-    args_.W2 = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
+    args_.W = args_.W2 = nullptr;  // FP64 p-form stage inputs (the mixed path keeps FP32 ones)
+    if (precision_ == kFP64) {
+        args_.W = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
+        args_.W2 = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
+    }
 
     Scalars s{};
     s.first_err = ~0ull;
@@ -179,6 +181,12 @@ void Engine::upload(const RunHost& r) {
 
     perm_ = L.perm;
     he_old_ = L.he_old;
+}
+
+void Engine::enqueue_prepare() {
+    if (precision_ == kMixed) mixed::enqueue_prepare(args_, stream_);
+    else fp64::enqueue_prepare(args_, stream_);
+    check(hipGetLastError(), "prepare launch");
 }
 
 void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev) {
@@ -282,6 +290,22 @@ void Engine::get_state(double* U_out) {
     synchronize();
     for (int32_t k = 0; k < nf_; ++k)
         for (int c = 0; c < dim_; ++c) U_out[size_t(perm_[k]) * dim_ + c] = U[size_t(k) * dim_ + c];
+}
+
+void Engine::set_state(const double* U_in, double t, int stop_bits) {
+    if (!U_in) throw std::invalid_argument("U_in is null");
+    synchronize();  // steps in flight still read the old state
+    std::vector<double> U(size_t(nf_) * dim_);
+    for (int32_t k = 0; k < nf_; ++k)
+        for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm_[k]) * dim_ + c];
+    check(hipMemcpy(args_.U, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice), "upload state");
+    Scalars s = read_scalars();
+    s.t = t;
+    s.stop = stop_bits;
+    s.first_err = ~0ull;
+    check(hipMemcpy(args_.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice), "upload scalars");
+    enqueue_prepare();  // stage inputs and dt partials of the new state
+    synchronize();
 }
 
 void Engine::diagnostics(double out[5]) {

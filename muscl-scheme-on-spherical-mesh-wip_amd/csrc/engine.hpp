@@ -50,6 +50,9 @@ public:
     int stop_bits();
     uint64_t first_error();
     void get_state(double* U_out);      // [nf][dim], slot 4 = energy
+    // resume: replaces the state (mesh face order), the time and the sticky
+    // stop bits; the next step's dt comes from the new state
+    void set_state(const double* U_in, double t, int stop_bits);
     void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
     void light_curves(double out[4]);   // get_light_curves (dim 5)
 
@@ -67,6 +70,7 @@ public:
 
 private:
     void enqueue_step(double dt0);
+    void enqueue_prepare();
     void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr);
     void init(const RunHost& r, const double* U_in);
     void upload(const RunHost& r);

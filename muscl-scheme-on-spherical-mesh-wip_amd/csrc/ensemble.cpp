@@ -175,13 +175,12 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         }
         args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars) * sc.size(), sc.data()));
         args_.U = static_cast<double*>(dev(sizeof(double) * U.size(), U.data()));
-        args_.W = static_cast<double*>(dev(sizeof(double) * U.size(), nullptr));
-        args_.W2 = nullptr;
+        args_.W = args_.W2 = nullptr;
         const size_t row = dim == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
 
-        fp64::enqueue_prepare(args_, stream_);
+        mixed::enqueue_prepare(args_, stream_);
         check(hipGetLastError(), "prepare launch");
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {

@@ -38,6 +38,7 @@ int block_size();
 }  // namespace fp64
 
 namespace mixed {
+void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
 int slots();      // LDS half-edge slots per block
 int max_faces();  // faces per block

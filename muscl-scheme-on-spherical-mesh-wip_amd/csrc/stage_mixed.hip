@@ -436,6 +436,45 @@ void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) 
 
 }  // namespace
 
+// E-form state -> FP32 stage input and this state's dt partials, with the
+// arithmetic of k_stage32's epilogue (face_terms_mixed), so a run started or
+// resumed from a state continues exactly as one that reached it by stepping.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_prepare32(GeomPtrs g, const PhysParams* __restrict__ Pv,
+                                                      const double* __restrict__ U, float* __restrict__ W32) {
+    const int b = xcd_block();
+    const PhysParams P = Pv[g.blk_run ? g.blk_run[b] : 0];
+    const int t = threadIdx.x;
+    if (t >= 64) return;  // a patch's faces fit one wave
+    const int i = g.blk_face[b] + t;
+    double vmax = 1e-8, dtx = 1e20;
+    if (i < g.blk_face[b + 1]) {
+        double u[DIM], w[DIM];
+        const double* ui = row(U, uint32_t(i * DIM));
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u[k] = ui[k];
+        const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
+        face_terms_mixed<DIM>(P, u, v3(c4.x, c4.y, c4.z), w, vmax, dtx);
+        store_w32<DIM>(W32, g.nf, i, w);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(vmax, o);
+        const double ox = __shfl_xor(dtx, o);
+        if (ov > vmax) vmax = ov;
+        if (dtx > ox) dtx = ox;
+    }
+    if (t == 0) {
+        g.dt_partial[2 * b] = vmax;
+        g.dt_partial[2 * b + 1] = dtx;
+    }
+}
+
+void enqueue_prepare(const StepArgs& a, hipStream_t s) {
+    if (a.phys.dim == 5) k_prepare32<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W32);
+    else k_prepare32<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W32);
+}
+
 int slots() { return kSlots; }
 int max_faces() { return kFaces; }
 

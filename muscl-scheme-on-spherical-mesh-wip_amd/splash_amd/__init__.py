@@ -218,6 +218,13 @@ class _MUSCL:
         check(lib().splash_diagnostics(self._ctx, _dp(out)))
         return out
 
+    def set_state(self, U, t, stop_bits=0):
+        """Resume from a checkpoint: state (mesh face order), time, stop bits."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (self._mesh.n_faces, self._dim):
+            raise ValueError(f"U must be [{self._mesh.n_faces}][{self._dim}], got {U.shape}")
+        check(lib().splash_set_state(self._ctx, _dp(U), float(t), int(stop_bits)))
+
     def get_light_curves(self):
         """adiabatic.hpp:228-333: [flux_0, flux_45, flux_90, flux_180]."""
         out = np.empty(4)

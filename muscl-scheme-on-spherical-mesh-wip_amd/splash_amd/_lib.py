@@ -67,6 +67,7 @@ SIGNATURES = {
     "splash_stop_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "splash_stop_bits": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "splash_get_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "splash_set_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int]),
     "splash_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "splash_light_curves": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "splash_num_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

@@ -69,3 +69,16 @@ def test_driver_loop_matches_oracle(tmp_path):
     g = S.Geometry(mesh)
     want = light_curves(ref.state(), g.fc, g.area, GAM)[0]
     assert lc[-1, 0] == pytest.approx(ref.time(), rel=1e-5) and lc[-1, 1] == pytest.approx(want, rel=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_checkpoint_round_trip(tmp_path, precision):
+    """write_checkpoint after 13 steps, 11 more steps; a second solver that
+    reads the checkpoint and steps 11 ends bit for bit in the same state."""
+    (tmp_path / "results").mkdir()
+    prog = os.path.join(ROOT, "tests", "cpp", "checkpoint_roundtrip")
+    r = subprocess.run([prog, str(precision), str(tmp_path / "ck.bin")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr

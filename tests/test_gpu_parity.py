@@ -234,3 +234,23 @@ def test_light_curves_device_reduction(precision):
     print(f"light curves {precision}: {got}")
     assert np.all(got > 0)
     np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_set_state_resumes_exactly(precision):
+    """Resuming from (state, time, stop bits) continues bit for bit as the
+    run that produced them."""
+    import splash_amd as S
+    mesh = S.quad_sphere(4)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=9, omega=(0, 0.2, 0.2))
+    prec = getattr(S, precision)
+    a = S.MUSCL_HLLC(mesh, U0, 5, GAM, 0.0, precision=prec)
+    a.do_steps(1e-4, 13)
+    U, t = a.get_state(), a.time()
+    bits, _ = a.stop_bits()
+    a.do_steps(1e-4, 11)
+    b = S.MUSCL_HLLC(mesh, U0, 5, GAM, 0.0, precision=prec)
+    b.set_state(U, t, bits)
+    b.do_steps(1e-4, 11)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert a.time() == b.time() and a.get_stop_check() == b.get_stop_check()
