@@ -280,16 +280,29 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     SlotRows r0;
     if (t < n_slots) r0 = load_slot(g, ext0 + uint32_t(t), edge_base);
     {
+        // a face set holds at most 1024 faces (4 per thread): every index is
+        // requested before the first gather, so each thread waits for two
+        // round trips, not two per face
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
         const float4* W4 = reinterpret_cast<const float4*>(W32);
-        for (int q = t; q < n_fs; q += kBlock) {
-            const uint32_t f = uint32_t(*row(g.fset_face, uint32_t(fs0 + q)));
-            const float4 w = *row(W4, f);
-            sW[q] = w;
-            if constexpr (DIM == 5) {
-                const float pq = *row(W32, uint32_t(4 * g.nf) + f);
-                sP[q] = pq;
-                sC[q] = m_sqrt(m_div(float(P.gam) * pq, w.x));
+        constexpr int kPer = 1024 / kBlock;
+        uint32_t f[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int q = t + j * kBlock;
+            f[j] = q < n_fs ? uint32_t(*row(g.fset_face, uint32_t(fs0 + q))) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int q = t + j * kBlock;
+            if (q < n_fs) {
+                const float4 w = *row(W4, f[j]);
+                sW[q] = w;
+                if constexpr (DIM == 5) {
+                    const float pq = *row(W32, uint32_t(4 * g.nf) + f[j]);
+                    sP[q] = pq;
+                    sC[q] = m_sqrt(m_div(float(P.gam) * pq, w.x));
+                }
             }
         }
     }

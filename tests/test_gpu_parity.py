@@ -254,3 +254,24 @@ def test_set_state_resumes_exactly(precision):
     b.do_steps(1e-4, 11)
     assert np.array_equal(a.get_state(), b.get_state())
     assert a.time() == b.time() and a.get_stop_check() == b.get_stop_check()
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+@pytest.mark.parametrize("column", [0, 4])
+def test_nan_state_stop_check_matches_oracle(precision, column):
+    """A NaN density reaches its neighbours' fluxes on the first step and
+    raises stop_check (MUSCL_base.hpp:264-268); a NaN energy does not: the
+    pressure floor max(1e-16, p) (adiabatic.hpp:582-592) turns it into 1e-16.
+    Both paths agree with the oracle on which."""
+    import splash_amd as S
+    mesh = S.icosphere_hex(3)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=2, omega=(0, 0, 0.2))
+    U0[17, column] = np.nan
+    s = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=getattr(S, precision))
+    s.do_step(1e-4)
+    ref = O.Solver(O.Mesh("hex", 3), U0, 5, O.HLLCPLUS, GAM, 0.0, _oparams(S.default_params()))
+    ref.do_step(1e-4)
+    assert s.get_stop_check() == ref.get_stop_check() == (column == 0)
+    if column == 0:
+        bits, _ = s.stop_bits()
+        assert bits & (1 | 2)
