@@ -23,6 +23,12 @@
 extern "C" {
 #endif
 
+/* The library is built with hidden symbol visibility; these declarations
+ * are its whole export list. */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
 #define SPLASH_OK 0
 #define SPLASH_EINVAL (-1)   /* bad argument */
 #define SPLASH_EMESH (-2)    /* mesh is not a closed manifold the stencil search accepts */
@@ -217,6 +223,10 @@ int splash_layout_stats(const splash_mesh_desc *mesh, int32_t out[5]);
 
 /* Human-readable text of the last error on this host thread. */
 const char *splash_last_error_message(void);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
 
 #ifdef __cplusplus
 }

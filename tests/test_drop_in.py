@@ -82,3 +82,12 @@ def test_checkpoint_round_trip(tmp_path, precision):
                        text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_ensemble_wrapper_matches_single_solvers(tmp_path):
+    (tmp_path / "results").mkdir()
+    prog = os.path.join(ROOT, "tests", "cpp", "ensemble_vs_single")
+    r = subprocess.run([prog], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
