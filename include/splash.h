@@ -146,6 +146,12 @@ int splash_diagnostics(splash_ctx *ctx, double out[5]);
  * z and -z, out = {flux_0, flux_45, flux_90, flux_180}; one device
  * reduction, no state transfer. */
 int splash_light_curves(splash_ctx *ctx, double out[4]);
+/* write_t_curl's per-face values (adiabatic.hpp:135-178, isothermal.hpp:
+ * 72-110), out[n_faces] in mesh face order: for face i,
+ * sum_j ((ec_j x L+_j) / -Sigma_i) . (v_j - v_j1) / A_i, with L+ the face's
+ * own reconstructed momenta of the last step's second stage (zero before a
+ * step or after splash_set_state, as in a freshly built reference solver). */
+int splash_curl(splash_ctx *ctx, double *out);
 int splash_num_faces(const splash_ctx *ctx, int32_t *n_faces, int32_t *n_halfedges);
 /* faces per stage-kernel patch in use (SPLASH_MIXED; the tuned value when
  * params.patch_faces = -1) */

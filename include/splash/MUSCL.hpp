@@ -246,6 +246,15 @@ public:
         for (auto& u : U) out_rho_ << u[0] << " ";
         out_rho_ << "\n";
     }
+    // adiabatic.hpp:135-178 / isothermal.hpp:72-110: face vorticity from the
+    // last step's reconstructed edge momenta, evaluated on the device
+    void write_t_curl() {
+        std::vector<double> c(mesh_.n_faces());
+        check_rc(splash_curl(ctx_, c.data()));
+        out_curl_ << time() << "  ";
+        for (double v : c) out_curl_ << v << " ";
+        out_curl_ << "\n";
+    }
 
 protected:
     MUSCL_gpu(const SurfaceMesh& mesh, const std::vector<std::vector<double>>& U_in, int dim, double gam,
@@ -286,7 +295,7 @@ protected:
     double rho_full_ = 0, E_full_ = 0;
     bool verbose_ = true;
     std::vector<double> fc_, area_;
-    std::ofstream out_rho_;
+    std::ofstream out_rho_, out_curl_;
 };
 
 // isothermal physics (isothermal.hpp): writers of p and omega_z per
@@ -305,7 +314,7 @@ protected:
                    int solver, const splash_params& p)
         : MUSCL_gpu(mesh, U_in, dim, gam, 0.0, solver, p) {
         out_p_.open("results/p.dat", std::ios::out | std::ios::trunc);
-        std::ofstream("results/curl.dat", std::ios::out | std::ios::trunc);
+        out_curl_.open("results/curl.dat", std::ios::out | std::ios::trunc);
         std::ofstream("results/omega.dat", std::ios::out | std::ios::trunc);
     }
     std::ofstream out_p_;
@@ -369,7 +378,7 @@ protected:
                   double omega_ns, int solver, const splash_params& p)
         : MUSCL_gpu(mesh, U_in, dim, gam, omega_ns, solver, p) {
         if (dim != 5) std::cout << "check dim \n";
-        std::ofstream("results/curl.dat", std::ios::out | std::ios::trunc);
+        out_curl_.open("results/curl.dat", std::ios::out | std::ios::trunc);
         out_p_.open("results/p.dat", std::ios::out | std::ios::trunc);
         out_omega_.open("results/omega.dat", std::ios::out | std::ios::trunc);
         const char* adrs[3] = {"results/Lx.dat", "results/Ly.dat", "results/Lz.dat"};

@@ -278,6 +278,15 @@ int splash_light_curves(splash_ctx* ctx, double out[4]) {
     });
 }
 
+int splash_curl(splash_ctx* ctx, double* out) {
+    CTX_OR_FAIL;
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        ctx->engine->curl(out);
+        return SPLASH_OK;
+    });
+}
+
 int splash_patch_faces(const splash_ctx* ctx, int32_t* patch_faces) {
     CTX_OR_FAIL;
     if (!patch_faces) return fail(SPLASH_EINVAL, "null output");

@@ -233,12 +233,14 @@ void Engine::profile(double dt0, int n, double* ms) {
 }
 
 void Engine::step(double dt0) {
+    stepped_ = true;
     enqueue_step(dt0);
     check(hipGetLastError(), "step launch");
 }
 
 void Engine::steps(double dt0, int n) {
     if (n <= 0) return;
+    stepped_ = true;
     constexpr int kGraphSteps = 8;
     if (n >= kGraphSteps) {
         if (!graph_exec_ || graph_dt0_ != dt0 || graph_steps_ != kGraphSteps) {
@@ -306,6 +308,7 @@ void Engine::set_state(const double* U_in, double t, int stop_bits) {
     check(hipMemcpy(args_.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice), "upload scalars");
     enqueue_prepare();  // stage inputs and dt partials of the new state
     synchronize();
+    stepped_ = false;
 }
 
 void Engine::diagnostics(double out[5]) {
@@ -332,6 +335,30 @@ void Engine::light_curves(double out[4]) {
     for (int k = 0; k < 4; ++k) out[k] = 0;
     for (int b = 0; b < n_diag_blocks_; ++b)
         for (int k = 0; k < 4; ++k) out[k] += part[4 * size_t(b) + k];
+}
+
+void Engine::curl(double* out) {
+    if (!out) throw std::invalid_argument("out is null");
+    if (!d_curl_) {
+        if (geo_.nh != nh_ || geo_.ev.size() != 3 * size_t(nh_))
+            throw std::logic_error("curl needs an Engine built from a mesh");
+        const size_t n = 3 * size_t(nh_) + size_t(nf_) * (1 + dim_);
+        std::vector<double> ev(3 * size_t(nh_));
+        for (int32_t h = 0; h < nh_; ++h)
+            for (int c = 0; c < 3; ++c) ev[3 * size_t(h) + c] = geo_.ev[3 * size_t(he_old_[h]) + c];
+        void* p = nullptr;
+        check(hipMalloc(&p, sizeof(double) * n), "hipMalloc");
+        allocations_.push_back(p);
+        d_curl_ = static_cast<double*>(p);
+        check(hipMemcpy(d_curl_, ev.data(), sizeof(double) * ev.size(), hipMemcpyHostToDevice), "upload edges");
+    }
+    double* d_out = d_curl_ + 3 * size_t(nh_);
+    fp64::enqueue_curl(args_, stepped_, d_curl_, d_out + nf_, d_out, stream_);
+    check(hipGetLastError(), "curl launch");
+    std::vector<double> v(nf_);
+    check(hipMemcpyAsync(v.data(), d_out, sizeof(double) * v.size(), hipMemcpyDeviceToHost, stream_), "read curl");
+    synchronize();
+    for (int32_t k = 0; k < nf_; ++k) out[perm_[k]] = v[k];
 }
 
 }  // namespace splash

@@ -55,6 +55,9 @@ public:
     void set_state(const double* U_in, double t, int stop_bits);
     void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
     void light_curves(double out[4]);   // get_light_curves (dim 5)
+    // write_t_curl's per-face values [nf] (mesh order) of the current state;
+    // needs an Engine built from a mesh
+    void curl(double* out);
 
     int32_t n_faces() const { return nf_; }
     int32_t n_halfedges() const { return nh_; }
@@ -95,6 +98,8 @@ private:
     double* d_light_w_ = nullptr;       // [nf][4] observer weights (dim 5)
     double* d_light_part_ = nullptr;    // [diag blocks][4]
     int n_diag_blocks_ = 0;
+    bool stepped_ = false;              // a step ran since the state was set
+    double* d_curl_ = nullptr;          // lazily: ev [nh][3], out [nf], W_tmp [nf][dim]
 
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;

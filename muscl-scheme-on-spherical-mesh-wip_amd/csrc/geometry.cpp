@@ -185,6 +185,7 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
     g.en.resize(3 * size_t(nh));
     g.ec.resize(3 * size_t(nh));
     g.ecn.resize(nh);
+    g.ev.resize(3 * size_t(nh));
     g.chord.resize(nh);
     g.arc.resize(nh);
     int missing = 0;
@@ -287,6 +288,7 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
             st(&g.ec[3 * size_t(h)], m);
             g.ecn[h] = norm(m);
             g.chord[h] = norm(vi - vi1);
+            st(&g.ev[3 * size_t(h)], vi - vi1);
             g.arc[h] = gc_distance(vi, vi1);
         }
     }

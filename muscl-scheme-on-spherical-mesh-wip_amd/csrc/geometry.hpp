@@ -36,6 +36,7 @@ struct Geometry {
     std::vector<double> ec;               // [nh][3] unit edge midpoint ((a+b)/2 normalised)
     std::vector<double> ecn;              // [nh]    |ec| as recomputed by vector3d::norm
     std::vector<double> chord;            // [nh]    |v_j - v_j1|
+    std::vector<double> ev;               // [nh][3] v_j - v_j1 (write_t_curl)
     std::vector<double> arc;              // [nh]    great-circle length of the edge
 
     double h0 = 0;                        // MUSCL_base.hpp:62-91

@@ -34,6 +34,10 @@ void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEven
 void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s);
 // per-block partial sums [ceil(nf/block_size())][4] of the light curves (dim 5)
 void enqueue_light(const StepArgs& a, const double* weights, double* partial, hipStream_t s);
+// write_t_curl per face [nf] from the last stage-2 input (W2, or W32b widened
+// into W_tmp [nf][dim]); stepped = false gives the value before any step.
+// ev = v_j - v_j1 per half-edge [nh][3].
+void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s);
 int block_size();
 }  // namespace fp64
 

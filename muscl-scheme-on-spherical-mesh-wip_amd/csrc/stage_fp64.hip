@@ -58,6 +58,22 @@ void enqueue_light(const StepArgs& a, const double* weights, double* partial, hi
     k_light_partial<<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, weights, partial);
 }
 
+void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s) {
+    const int gf = (a.geom.nf + kBlock - 1) / kBlock;
+    const double* W = nullptr;
+    if (stepped) {
+        if (a.W2) {
+            W = a.W2;
+        } else {  // FP32 stage-2 input, widened
+            if (a.phys.dim == 5) k_widen32<5><<<gf, kBlock, 0, s>>>(a.geom.nf, a.W32b, W_tmp);
+            else k_widen32<4><<<gf, kBlock, 0, s>>>(a.geom.nf, a.W32b, W_tmp);
+            W = W_tmp;
+        }
+    }
+    if (a.phys.dim == 5) k_curl<5><<<gf, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, W, ev, out);
+    else k_curl<4><<<gf, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, W, ev, out);
+}
+
 int block_size() { return kBlock; }
 
 }  // namespace fp64

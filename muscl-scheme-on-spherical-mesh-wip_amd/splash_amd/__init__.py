@@ -231,6 +231,13 @@ class _MUSCL:
         check(lib().splash_light_curves(self._ctx, _dp(out)))
         return out
 
+    def curl(self):
+        """write_t_curl's per-face values (adiabatic.hpp:135-178,
+        isothermal.hpp:72-110) of the current state, [n_faces]."""
+        out = np.empty(self._mesh.n_faces)
+        check(lib().splash_curl(self._ctx, _dp(out)))
+        return out
+
     def patch_faces(self):
         """Faces per stage-kernel patch (mixed precision; 0 for FP64)."""
         n = C.c_int32()

@@ -70,6 +70,7 @@ SIGNATURES = {
     "splash_set_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int]),
     "splash_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "splash_light_curves": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "splash_curl": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "splash_num_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "splash_face_geometry": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "splash_geometry_create": (C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_void_p)]),

@@ -85,6 +85,7 @@ void or_do_step(or_solver *s, double dt0);                     /* MUSCL_base.hpp
 double or_time(const or_solver *s);                            /* :227 */
 int or_stop_check(const or_solver *s);                         /* :232 */
 void or_get_state(const or_solver *s, double *U_out);          /* [nf][dim], slot 4 = E */
+void or_curl(const or_solver *s, double *out);                 /* adiabatic.hpp:135-178, [nf] */
 double or_last_dt(const or_solver *s);
 void or_diagnostics(const or_solver *s, double out[5]);        /* :187-205 */
 double or_area_coeff(const or_solver *s);                      /* adiabatic.hpp:70-84 */

@@ -697,6 +697,28 @@ double or_time(const or_solver *s) { return s->t; }
 int or_stop_check(const or_solver *s) { return s->stop; }
 double or_last_dt(const or_solver *s) { return s->dt; }
 double or_area_coeff(const or_solver *s) { return s->area_coeff; }
+/* write_t_curl (adiabatic.hpp:135-178, isothermal.hpp:72-110): per face,
+ * sum over its edges of ((edge_center x L+) / (-Sigma)) . (v_j - v_j1) / A,
+ * with L+ the face's own reconstructed state of the last find_U_edges. */
+void or_curl(const or_solver *s, double *out)
+{
+    const or_mesh *m = &s->mesh;
+    for (int f = 0; f < s->nf; f++) {
+        const int n = m->foff[f + 1] - m->foff[f];
+        double vort = 0;
+        for (int j = 0; j < n; j++) {
+            const double *up = s->Uplus + (size_t)(m->foff[f] + j) * s->dim;
+            v3 l = {up[1], up[2], up[3]};
+            const int j1 = (j == n - 1) ? 0 : j + 1;
+            v3 ec = edge_center(s, f, j);
+            v3 vel = v3divs(v3cross(ec, l), -s->U[(size_t)f * s->dim]);
+            v3 r = v3sub(v3ld(m->xyz + 3 * m->fv[m->foff[f] + j]), v3ld(m->xyz + 3 * m->fv[m->foff[f] + j1]));
+            vort += v3dot(vel, r);
+        }
+        out[f] = vort / s->g->area[f];
+    }
+}
+
 void or_get_state(const or_solver *s, double *U_out)
 {
     memcpy(U_out, s->U, sizeof(double) * (size_t)s->nf * s->dim);

@@ -71,11 +71,16 @@ int main(int argc, char** argv) {
         if (steps % skipstep == 0) {
             test2.write_t_rho();
             test2.write_t_p();
+            test2.write_t_curl();
             test2.write_t_omega_z();
             test2.write_t_L();
         }
         if (test2.get_stop_check()) {
             test2.write_t_rho();
+            test2.write_t_p();
+            test2.write_t_curl();
+            test2.write_t_omega_z();
+            test2.write_t_L();
             break;
         }
         steps++;

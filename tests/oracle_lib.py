@@ -73,6 +73,7 @@ def lib():
         L.or_stop_check.argtypes = [C.c_void_p]
         L.or_get_state.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.or_diagnostics.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.or_curl.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_limiter_adiabatic.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double,
                                            C.c_double, C.c_int, C.POINTER(C.c_double)]
@@ -188,6 +189,11 @@ class Solver:
     def diagnostics(self):
         out = np.empty(5)
         lib().or_diagnostics(self.ptr, _dp(out))
+        return out
+
+    def curl(self):
+        out = np.empty(self.mesh.nf)
+        lib().or_curl(self.ptr, _dp(out))
         return out
 
     def __del__(self):

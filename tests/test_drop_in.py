@@ -56,11 +56,19 @@ def test_driver_loop_matches_oracle(tmp_path):
     # text outputs carry 6 significant digits (std::ostream default)
     final = np.loadtxt(tmp_path / "results" / "final_state.dat")
     ref = O.Solver(O.Mesh("quad", 4), np.loadtxt(tmp_path / "input" / "input.dat"), 5, O.HLLCPLUS, GAM, 0.0)
-    for _ in range(PARAMS["maxstep"]):
+    curl_rows = []
+    for k in range(PARAMS["maxstep"]):
         ref.do_step(PARAMS["dt"])
+        if k % PARAMS["skipstep"] == 0:
+            curl_rows.append(np.concatenate([[ref.time()], ref.curl()]))
     assert np.allclose(final, ref.state(), rtol=1e-5, atol=1e-6 * np.max(np.abs(ref.state())))
     rho = np.loadtxt(tmp_path / "results" / "rho.dat")
     assert rho.shape == (1 + PARAMS["maxstep"] // PARAMS["skipstep"], 1 + mesh.n_faces)
+    # write_t_curl after every skipstep-th step, as test.cpp:123-131
+    curl = np.loadtxt(tmp_path / "results" / "curl.dat")
+    want = np.array(curl_rows)
+    assert curl.shape == want.shape
+    assert np.allclose(curl, want, rtol=1e-5, atol=1e-5 * np.max(np.abs(want[:, 1:])))
     fc = np.loadtxt(tmp_path / "results" / "face_centers.dat")
     assert fc.shape == (mesh.n_faces, 3) and np.allclose(np.linalg.norm(fc, axis=1), 1, atol=1e-14)
     # the light curve written every step (get_light_curves on the device)

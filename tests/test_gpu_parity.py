@@ -275,3 +275,41 @@ def test_nan_state_stop_check_matches_oracle(precision, column):
     if column == 0:
         bits, _ = s.stop_bits()
         assert bits & (1 | 2)
+
+
+@pytest.mark.parametrize("solver,kind,args", [("HLLCplus", "hex", (4,)), ("HLLC", "quad", (4,)),
+                                              ("HLLE", "hex", (4,))])
+def test_curl_matches_oracle(solver, kind, args):
+    """write_t_curl's per-face values: zero before a step and after set_state
+    (the reference's freshly sized U_plus); after steps the FP64 path matches
+    the oracle to 1e-10 of the largest magnitude and the FP32-flux path to
+    2e-4 of it."""
+    import splash_amd as S
+    mesh = _gen(S, kind)(*args)
+    fc = S.ic.unit_centres(mesh.face_centres())
+    if solver == "HLLE":
+        U0 = S.ic.isothermal_state(mesh.face_centres(), seed=3)
+        v = np.cross([0.1, 0.2, 0.3], fc)
+        U0[:, 1:4] = U0[:, :1] * np.cross(fc, v)
+    else:
+        U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=3, omega=(0.1, 0.2, 0.3))
+    gpu, ref = _pair(S, kind, args, solver, U0, 0, 1e-4)
+    assert np.array_equal(gpu.curl(), np.zeros(mesh.n_faces))
+    for _ in range(20):
+        gpu.do_step(1e-4)
+        ref.do_step(1e-4)
+    want = ref.curl()
+    scale = np.max(np.abs(want))
+    err = np.max(np.abs(gpu.curl() - want)) / scale
+    cls = getattr(S, "MUSCL_" + solver)
+    if solver == "HLLE":
+        mixed = cls(mesh, U0, 4, GAM, precision=S.MIXED)
+    else:
+        mixed = cls(mesh, U0, 5, GAM, 0.0, False, precision=S.MIXED)
+    mixed.do_steps(1e-4, 20)
+    err32 = np.max(np.abs(mixed.curl() - want)) / scale
+    print(f"curl {solver} {kind}{args}: |curl| max {scale:.3e}, FP64 err {err:.3e}, mixed err {err32:.3e}")
+    assert err <= TOL
+    assert err32 <= 2e-4
+    gpu.set_state(gpu.get_state(), gpu.time())
+    assert np.array_equal(gpu.curl(), np.zeros(mesh.n_faces))
