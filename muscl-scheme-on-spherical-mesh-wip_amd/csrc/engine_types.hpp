@@ -70,7 +70,7 @@ struct GeomPtrs {
     //     x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30
     //     y = minus1 | owner << 10 | edge << 20 | valence << 28
     //     (face fields: face-set locals)
-    //   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
+    //   slot_b [n_ext][4]: H+/H-, BM_dist/H+, H+/BM_dist, arc/area
     // and per edge of a block (n_slots/2 edges from row (foff[f0] +
     // halo_off[b]) / 2, oriented for the slot that evaluates the flux):
     //   edge32 [n_ext/2][8]: unit edge midpoint xyz, |ec|, normal xyz, chord

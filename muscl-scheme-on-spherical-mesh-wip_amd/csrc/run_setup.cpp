@@ -234,9 +234,10 @@ MixedTables pack_mixed_tables(const RunHost& r) {
             const float bp = float(r.beta[4 * hn + 0]), bm = float(r.beta[4 * hn + 2]);
             std::memcpy(&m.slot_a[4 * e + 2], &bp, 4);
             std::memcpy(&m.slot_a[4 * e + 3], &bm, 4);
-            m.slot_b[4 * e + 0] = float(1.0 / r.hsc[4 * hn + 0]);
-            m.slot_b[4 * e + 1] = float(1.0 / r.hsc[4 * hn + 1]);
-            m.slot_b[4 * e + 2] = float(r.hsc[4 * hn + 2]);
+            const double h_p = r.hsc[4 * hn + 0], h_m = r.hsc[4 * hn + 1], bm_dist = r.hsc[4 * hn + 2];
+            m.slot_b[4 * e + 0] = float(h_p / h_m);
+            m.slot_b[4 * e + 1] = float(bm_dist / h_p);
+            m.slot_b[4 * e + 2] = float(h_p / bm_dist);
             m.slot_b[4 * e + 3] = float(r.hsc[4 * hn + 3]);
             if (rep) {
                 float* row = &m.edge32[8 * (e0 / 2 + ei)];

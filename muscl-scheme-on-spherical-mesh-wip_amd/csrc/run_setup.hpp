@@ -59,7 +59,7 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces);
 //                      y = minus1 | owner << 10 | edge << 20 | valence << 28
 //                      (face-set locals), then beta+0, beta-0 as float bits
 //                      (beta.1 = 1 - beta.0 to 3e-13 on every generated mesh)
-//   slot_b [n_ext][4]: 1/H+, 1/H-, BM_dist, arc/area
+//   slot_b [n_ext][4]: H+/H-, BM_dist/H+, H+/BM_dist, arc/area
 //   edge32 [n_ext/2][8]: unit edge midpoint, |ec|, normal, chord
 //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower-index << 18
 //   rate, rd_area [nf]: accretion rate profile, round_diff / area

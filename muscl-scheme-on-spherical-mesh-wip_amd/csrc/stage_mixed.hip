@@ -33,7 +33,7 @@ __device__ __forceinline__ float m_tanh(float x) {
 // float4 (Sigma, L) row per face plus (dim 5) one pressure per face, staged
 // per block in LDS, and four float4 rows plus one packed 8-byte record per
 // slot.  The state, the RK combine, the dt limits and reductions and the
-// flux accumulation (compensated, TwoSum) stay FP64; the accretion /
+// flux accumulation (Kahan-compensated) stay FP64; the accretion /
 // cooling / friction sources run in FP32 on the FP32 stage input.
 
 // Patches: at most kFaces faces (one wave updates them) whose own
@@ -60,11 +60,58 @@ __device__ __forceinline__ void store_w32(float* __restrict__ W32, int nf, int i
     if constexpr (DIM == 5) *row(W32, uint32_t(4 * nf + i)) = float(w[4]);
 }
 
+// h(r) of adiabatic.hpp:685-828 in FP32: 1 - tanh(a) for 0 < r < 1 with
+// a = 4 r^2 (1-r)^0.1, tanh(a)^6 for r >= 1 with a = 10 (r-1)^3, else 0.
+// Both arguments are >= 0, so tanh(a) = 1 - 2q with q = 1 / (e^{2a} + 1),
+// and 2a/ln 2 is formed directly as the exponent of one exp2 (the constant
+// 8/ln 2 of the first enters as log2(8/ln 2) under the power).
+__device__ __forceinline__ float h_switch32(float r) {
+    const bool mid = r < 1.0f && r > 0.0f;
+    const bool high = r >= 1.0f;
+    const float x = r - 1.0f;
+    const float y = mid ? (r * r) * __builtin_amdgcn_exp2f(fmaf(0.1f, __builtin_amdgcn_logf(-x), 3.5287663729448977f))
+                        : (x * x) * (x * 28.853900817779268f);
+    const float q = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(y) + 1.0f);
+    const float t = fmaf(-2.0f, q, 1.0f), t2 = t * t;
+    return mid ? 2.0f * q : (high ? t2 * t2 * t2 : 0.0f);
+}
+
+// The limiters of adiabatic.hpp:685-828 and isothermal.hpp:269-320 on
+// r = rho * D (rho = H+/H-, D = (w - u-) / (u+ - w)), with e = H+/BM =
+// eta_plus: eta_minus * r = e * D, and eta_minus = e / rho.
+template <int DIM>
+__device__ __forceinline__ void limiter32(const float* D, float rho, float e, float nu, int valence, float* lim) {
+    if constexpr (DIM == 5) {
+        const float ed = e * __builtin_amdgcn_rcpf(float(2 * valence) * nu);  // eta_minus r / den = ed D
+        const float c3 = (1.0f + nu) * (1.0f / 3.0f);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            const float rk = rho * D[k];
+            const float sb = lmax(0.0f, lmax(lmin(1.0f, D[k] * ed), lmin(rk, e)));
+            const float to = lmax(0.0f, lmin(sb, fmaf(c3, rk - 1.0f, 1.0f)));
+            const float l = fmaf(h_switch32(rk), sb - to, to);  // (1-h) third_order + h superbee
+            lim[k] = __builtin_isnan(rk) ? 0.0f : l;
+        }
+    } else {
+        const float ed = e * __builtin_amdgcn_rcpf(float(valence) * nu);  // 2 eta r / den = ed D
+        const float eta = e * __builtin_amdgcn_rcpf(rho);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            const float rk = rho * D[k];
+            const float l = lmax(0.0f, lmax(lmin(1.0f, D[k] * ed), lmin(rk, eta)));
+            lim[k] = __builtin_isnan(rk) ? 0.0f : l;
+        }
+    }
+}
+
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
-// block's LDS face states (sW: Sigma, L; sP: pressure): li = owner, plus0,
-// plus1, minus0, minus1 local indices; bp, bm = beta+0, beta-0 (the pair's
-// second weight is 1 - the first); hs = {1/H+, 1/H-, BM, arc/area}; e4 =
-// {ec, |ec|}; en = the half-edge's outward normal.
+// block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed): li
+// = owner, plus0, plus1, minus0, minus1 local indices; bp, bm = beta+0,
+// beta-0 (the pair's second weight is 1 - the first); hs = {H+/H-, BM/H+,
+// H+/BM, arc/area}; e4 = {ec, |ec|}; en = the half-edge's outward normal.
+// The slopes stay raw differences d+ = u+ - w, d- = w - u-: the reference's
+// r = (d-/H-) / (d+/H+) is (H+/H-) d-/d+ and its U+ = w + (d+/H+) lim BM is
+// w + d+ (BM/H+) lim.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, const float* sP,
                                                 const float* sC, const int* li, int valence, float bp, float bm,
@@ -77,31 +124,29 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         if constexpr (DIM == 5) w[q][4] = sP[li[q]];
     }
     const float* wi = w[0];
-    float pp[DIM], r[DIM], lim[DIM];
+    float dp[DIM], D[DIM], lim[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-        pp[k] = ((w[2][k] + bp * (w[1][k] - w[2][k])) - wi[k]) * hs.x;
-        const float pm = (wi[k] - (w[4][k] + bm * (w[3][k] - w[4][k]))) * hs.y;
-        r[k] = m_div(pm, pp[k]);
+        dp[k] = fmaf(bp, w[1][k] - w[2][k], w[2][k]) - wi[k];
+        const float dm = wi[k] - fmaf(bm, w[3][k] - w[4][k], w[4][k]);
+        D[k] = dm * __builtin_amdgcn_rcpf(dp[k]);
     }
     const V3<float> ec = v3(e4.x, e4.y, e4.z);
-    const float eta_m = __builtin_amdgcn_rcpf(hs.z * hs.y), eta_p = __builtin_amdgcn_rcpf(hs.z * hs.x);  // H-+/BM
+    // v . en with v = (ec x L) / (-Sigma |ec|) (dim 5) or (ec x L) / -Sigma
+    const float cl = dot(cross(ec, v3(wi[1], wi[2], wi[3])), en);
     if constexpr (DIM == 5) {
-        const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0] * e4.w);
-        const float cs = sC[li[0]];  // the owner's sound speed, staged with its state
-        const float nu = (cs + dot(vel, en)) * dt * hs.w;
-        limiter_adiabatic<float, DIM>(r, nu, eta_m, eta_p, valence, lim);
+        const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0] * e4.w)) * dt * hs.w;
+        limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
     } else {
-        const V3<float> vel = cross(ec, v3(wi[1], wi[2], wi[3])) / (-wi[0]);
-        const float nu = (1.0f + dot(vel, en)) * dt * hs.w;
-        limiter_isothermal<float, DIM>(r, nu, eta_m, valence, lim);
+        const float nu = (1.0f - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
+        limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
     }
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) out[k] = wi[k] + pp[k] * lim[k] * hs.z;
+    for (int k = 0; k < DIM; ++k) out[k] = fmaf(dp[k] * hs.y, lim[k], wi[k]);
     if constexpr (DIM == 5) {
-        // E_fc (MUSCL_base.hpp:541-562)
-        const V3<float> vel = cross(ec, v3(out[1], out[2], out[3])) / (-out[0]);
-        out[4] = out[4] * float(1 / (P.gam - 1)) + out[0] * dot(vel, vel) * 0.5f;
+        // E_fc (MUSCL_base.hpp:541-562): Sigma |v|^2 / 2 = |ec x L|^2 / (2 Sigma)
+        const V3<float> c = cross(ec, v3(out[1], out[2], out[3]));
+        out[4] = fmaf(out[4], float(1 / (P.gam - 1)), dot(c, c) * (0.5f * __builtin_amdgcn_rcpf(out[0])));
     }
 }
 
@@ -202,8 +247,8 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
     if (cs > vmax) vmax = cs;
 }
 
-// One slot's rows: packed record (x, y, beta+0, beta-0), {1/H+, 1/H-, BM,
-// arc/area} and its edge's {ec, |ec|}, {en, chord}.
+// One slot's rows: packed record (x, y, beta+0, beta-0), {H+/H-, BM/H+,
+// H+/BM, arc/area} and its edge's {ec, |ec|}, {en, chord}.
 struct SlotRows {
     uint4 ra;
     float4 hs, e4, n4;
@@ -375,21 +420,22 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
                 for (int k = 0; k < 5; ++k) res[k] = dt_here * double(src[k]);
             }
         }
-        double sum[DIM], comp[DIM];
+        // Kahan-compensated FP64 sum of the FP32 edge terms: c carries the
+        // (negated) low part each addition loses into the next term.  The
+        // terms are FP32 numbers, so the partial sums are exact, and c stays
+        // 0, unless a face's terms span more than 2^26 in magnitude.
+        double sum[DIM], c[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) sum[k] = comp[k] = 0;
+        for (int k = 0; k < DIM; ++k) sum[k] = c[k] = 0;
         bool nan_flux = false;
         for (int hh = hb; hh < he; ++hh) {
             const float* Fh = &sF[(hh - hbase) * DIM];
 #pragma unroll
             for (int k = 0; k < DIM; ++k) {
-                // compensated sum: TwoSum's exact rounding error of each
-                // addition goes into comp
                 nan_flux |= __builtin_isnan(Fh[k]);
-                const double x = double(Fh[k]);
-                const double s = sum[k] + x;
-                const double xb = s - sum[k];
-                comp[k] += (sum[k] - (s - xb)) + (x - xb);
+                const double y = double(Fh[k]) - c[k];
+                const double s = sum[k] + y;
+                c[k] = (s - sum[k]) - y;
                 sum[k] = s;
             }
         }
@@ -397,7 +443,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         const double coef = rd_area * dt_here;
         double un[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[k] = (res[k] - coef * (sum[k] + comp[k])) + u0[k];
+        for (int k = 0; k < DIM; ++k) un[k] = (res[k] - coef * (sum[k] - c[k])) + u0[k];
         if (un[0] < 1e-8) un[0] = 1e-8;
         double wn[DIM];
         face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
