@@ -283,8 +283,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
                                                        Scalars* __restrict__ scv, double* __restrict__ U,
                                                        const float* __restrict__ W32,
                                                        float* __restrict__ W32_out) {
+    // reconstructed half-edge states; after the flux phase has read them,
+    // the same rows hold each own slot's flux times chord length
     __shared__ float sUP[kSlots * DIM];
-    __shared__ float sF[kSlots * DIM];  // edge flux times chord length, per own slot
+    float* const sF = sUP;
     const int b = xcd_block();
     const int run = g.blk_run ? g.blk_run[b] : 0;  // ensembles: this patch's run
     const PhysParams P = Pv[run];
@@ -298,23 +300,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     const double dt = sc->dt;
     const uint32_t ext0 = uint32_t(hbase + hoff), edge_base = ext0 >> 1;
 
-    // face data of phase 3, requested before the stencil gathers
     const int i = f0 + t;
     const bool face = i < f1;  // t < kFaces
-    double u0[DIM], rd_area = 0, rate = 0;
-    V3<double> fcn = v3(0.0, 0.0, 0.0);
-    int hb = 0, he = 0;
-    if (face) {
-        hb = *row(g.foff, uint32_t(i));
-        he = *row(g.foff, uint32_t(i + 1));
-        const double* ui = row(U, uint32_t(i * DIM));
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
-        const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
-        fcn = v3(c4.x, c4.y, c4.z);
-        rd_area = *row(g.rd_area, uint32_t(i));
-        if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
-    }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
     // faces first, so face f0 + t sits at local index t); the first slot's
@@ -363,6 +350,22 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         if (t < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r0, &sUP[t * DIM]);
         if (t1 < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r1, &sUP[t1 * DIM]);
     }
+    // face data of phase 3, requested here so the flux phase hides the
+    // loads without holding them through the reconstruction
+    double u0[DIM], rd_area = 0, rate = 0;
+    V3<double> fcn = v3(0.0, 0.0, 0.0);
+    int hb = 0, he = 0;
+    if (face) {
+        hb = *row(g.foff, uint32_t(i));
+        he = *row(g.foff, uint32_t(i + 1));
+        const double* ui = row(U, uint32_t(i * DIM));
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
+        const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
+        fcn = v3(c4.x, c4.y, c4.z);
+        rd_area = *row(g.rd_area, uint32_t(i));
+        if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
+    }
     uint32_t rec = 0;
     float4 e4, n4;
     if (t < n_edges) {
@@ -377,15 +380,16 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     // edge's lower-index half-edge (the twin side has the same midpoint and
     // the negated normal) and written as +F*chord / -F*chord, so the two
     // faces of an edge exchange exactly opposite fluxes
+    const int sw = int(rec & 511u);          // the evaluating (own) slot
+    const int tws = int((rec >> 9) & 511u);  // its twin's slot
+    const bool low = (rec >> 18) & 1u;
+    float F[DIM];
     if (t < n_edges) {
-        const int sw = int(rec & 511u);          // the evaluating (own) slot
-        const int tws = int((rec >> 9) & 511u);  // its twin's slot
-        const bool low = (rec >> 18) & 1u;
         Edge<float> e;
         e.ec = v3(e4.x, e4.y, e4.z);
         e.ecn = e4.w;
         e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
-        float uL[DIM], uR[DIM], F[DIM];
+        float uL[DIM], uR[DIM];
         const int sl = low ? sw : tws, sr = low ? tws : sw;
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
@@ -396,10 +400,15 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         if (err) raise_stop(sc, err, hbase + sw);
         const float chord = low ? n4.w : -n4.w;
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) sF[sw * DIM + k] = F[k] * chord;
+        for (int k = 0; k < DIM; ++k) F[k] *= chord;
+    }
+    __syncthreads();  // every edge's states are read: sF may overwrite them
+    if (t < n_edges) {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) sF[sw * DIM + k] = F[k];
         if (low && tws < n_own)
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) sF[tws * DIM + k] = -(F[k] * chord);
+            for (int k = 0; k < DIM; ++k) sF[tws * DIM + k] = -F[k];
     }
     __syncthreads();
 
@@ -483,7 +492,7 @@ namespace {
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
     const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 24 : 16);
-    if (a.geom.fset_max > 1024 || lds + 2 * sizeof(float) * kSlots * DIM > 64 * 1024)
+    if (a.geom.fset_max > 1024 || lds + sizeof(float) * kSlots * DIM > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
         launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
