@@ -245,6 +245,9 @@ MixedTables pack_mixed_tables(const RunHost& r) {
                     row[c] = float(r.ec[4 * hn + c]);
                     row[4 + c] = float(r.en[4 * hn + c]);
                 }
+                // the FP32 kernel takes |ec| (the recomputed norm of the unit
+                // midpoint) as exactly 1
+                if (row[3] != 1.0f) throw std::runtime_error("edge midpoint norm is not 1 in FP32 (degenerate edge)");
             }
         }
     }

@@ -132,10 +132,11 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         D[k] = dm * __builtin_amdgcn_rcpf(dp[k]);
     }
     const V3<float> ec = v3(e4.x, e4.y, e4.z);
-    // v . en with v = (ec x L) / (-Sigma |ec|) (dim 5) or (ec x L) / -Sigma
+    // v . en with v = (ec x L) / (-Sigma |ec|) (dim 5; |ec| is 1 in FP32) or
+    // (ec x L) / -Sigma
     const float cl = dot(cross(ec, v3(wi[1], wi[2], wi[3])), en);
     if constexpr (DIM == 5) {
-        const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0] * e4.w)) * dt * hs.w;
+        const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
         limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
     } else {
         const float nu = (1.0f - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
@@ -147,6 +148,122 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         // E_fc (MUSCL_base.hpp:541-562): Sigma |v|^2 / 2 = |ec x L|^2 / (2 Sigma)
         const V3<float> c = cross(ec, v3(out[1], out[2], out[3]));
         out[4] = fmaf(out[4], float(1 / (P.gam - 1)), dot(c, c) * (0.5f * __builtin_amdgcn_rcpf(out[0])));
+    }
+}
+
+// flux_star in FP32 (isothermal.hpp:141-260, HLLE_p.hpp:34-55, HLLC.hpp:
+// 18-122, HLLCplus.hpp:17-177): the formulas, branches and stop codes of the
+// FP64 flux_star, with |ec| = 1 (the FP64 unit midpoint's norm rounds to
+// exactly 1.0f, checked when the tables are packed, so the divisions by it
+// drop out), one reciprocal of each side's Sigma, Sigma |v|^2 without the
+// square root, and HLLC+'s velocity-jump factors formed before the star
+// flux so the two side velocities are dead by then.  ec, en: unit edge
+// midpoint and the normal of the left state's face.
+template <int DIM, int SOLVER>
+__device__ __forceinline__ int flux32(float gam, const float* uL, const float* uR, V3<float> ec, V3<float> en,
+                                      float* F) {
+    const float iL = __builtin_amdgcn_rcpf(uL[0]), iR = __builtin_amdgcn_rcpf(uR[0]);
+    const V3<float> vL = cross(ec, v3(uL[1], uL[2], uL[3])) * (-iL);  // (ec x L) / -Sigma
+    const V3<float> vR = cross(ec, v3(uR[1], uR[2], uR[3])) * (-iR);
+    const float VL = dot(en, vL), VR = dot(en, vR);
+    if constexpr (SOLVER == splash::kHLLE) {
+        // sound speed 1, PI = Sigma
+        const float SL = smin(VL, VR) - 1.0f, SR = smax(VL, VR) + 1.0f;
+        const V3<float> nxR = cross(en, ec);
+        const float FL[4] = {uL[0] * VL, uL[1] * VL - nxR.x * uL[0], uL[2] * VL - nxR.y * uL[0],
+                             uL[3] * VL - nxR.z * uL[0]};
+        const float FR[4] = {uR[0] * VR, uR[1] * VR - nxR.x * uR[0], uR[2] * VR - nxR.y * uR[0],
+                             uR[3] * VR - nxR.z * uR[0]};
+        return hlle_combine<float, 4>(SL, SR, uL, uR, FL, FR, F) ? splash::kStopWaveBranch : 0;
+    } else {
+        int err = 0;
+        const float pL = smax(1e-16f, (uL[4] - uL[0] * dot(vL, vL) * 0.5f) * (gam - 1.0f));
+        const float pR = smax(1e-16f, (uR[4] - uR[0] * dot(vR, vR) * 0.5f) * (gam - 1.0f));
+        const float aL = m_sqrt(gam * pL * iL), aR = m_sqrt(gam * pR * iR);
+        // PVRS wave speeds (adiabatic.hpp:594-683)
+        const float p_star = smax(0.0f, 0.5f * (pL + pR) - 0.125f * (VR - VL) * (uL[0] + uR[0]) * (aL + aR));
+        const float kq = (gam + 1.0f) * __builtin_amdgcn_rcpf(2.0f * gam);
+        float qL = 1.0f, qR = 1.0f;
+        if (!(p_star <= pR)) {
+            qR = m_sqrt(1.0f + kq * (p_star * __builtin_amdgcn_rcpf(pR) - 1.0f));
+            if (m_isnan(qR) || m_isinf(qR)) qR = 1.0f;
+        }
+        if (!(p_star <= pL)) {
+            qL = m_sqrt(1.0f + kq * (p_star * __builtin_amdgcn_rcpf(pL) - 1.0f));
+            if (m_isnan(qL) || m_isinf(qL)) qL = 1.0f;
+        }
+        const float SL = VL - aL * qL, SR = VR + aR * qR;
+        if (m_isnan(SL) || m_isnan(SR)) err |= splash::kStopNaNWaveSpeed;
+        if constexpr (SOLVER == splash::kHLLE_P) {
+            const V3<float> nxR = cross(en, ec);
+            float FL[5], FR[5];
+            flux_adiabatic(uL, vL, pL, nxR, Edge<float>{ec, 1.0f, en}, FL);
+            flux_adiabatic(uR, vR, pR, nxR, Edge<float>{ec, 1.0f, en}, FR);
+            if (hlle_combine<float, 5>(SL, SR, uL, uR, FL, FR, F)) err |= splash::kStopWaveBranch;
+            return err;
+        } else {
+            const float phi_L = uL[0] * (SL - VL), phi_R = uR[0] * (SR - VR);
+            float S_star = (pR - pL + phi_L * VL - phi_R * VR) * __builtin_amdgcn_rcpf(phi_L - phi_R);
+            if (m_isnan(S_star) || m_isinf(S_star)) S_star = (VL + VR) * 0.5f;
+            const float P_LR = (pL + pR + phi_L * (S_star - VL) + phi_R * (S_star - VR)) * 0.5f;
+            // HLLCplus.hpp:123-165 factors: f* - 1, 1 - (p ratio)^M, the
+            // velocity jump, V_R - V_L
+            float fm1 = 0.0f, g = 0.0f;
+            V3<float> dvel = v3(0.0f, 0.0f, 0.0f);
+            if constexpr (SOLVER == splash::kHLLCPLUS) {
+                const float M = smin(1.0f, smax(m_sqrt(dot(vL, vL)) * __builtin_amdgcn_rcpf(aL),
+                                                m_sqrt(dot(vR, vR)) * __builtin_amdgcn_rcpf(aR)));
+                const bool lambd = ((VL - aL) > 0.0f && (VR - aR) < 0.0f) || ((VL + aL) > 0.0f && (VR + aR) < 0.0f) ||
+                                   ((VR - aR) > 0.0f && (VL - aL) < 0.0f) || ((VR + aR) > 0.0f && (VL + aL) < 0.0f);
+                const float omm = 1.0f - M * M;
+                const float f_star = lambd ? 1.0f : M * m_sqrt(4.0f + omm * omm) * __builtin_amdgcn_rcpf(1.0f + M * M);
+                fm1 = f_star - 1.0f;
+                const float hh = smin(pL * __builtin_amdgcn_rcpf(pR), pR * __builtin_amdgcn_rcpf(pL));
+                g = 1.0f - m_pow(hh, M);
+                dvel = vR - vL;
+            }
+            // branch of HLLC.hpp:98-119
+            int side;  // 0: F_L, 1: F_L*, 2: F_R*, 3: F_R
+            if (SL >= 0.0f) side = 0;
+            else if (SL < 0.0f && S_star >= 0.0f) side = 1;
+            else if (S_star < 0.0f && SR >= 0.0f) side = 2;
+            else if (SR < 0.0f) side = 3;
+            else {
+                side = 3;
+                err |= splash::kStopWaveBranch;
+            }
+            const bool left = side <= 1;
+            float u[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) u[k] = left ? uL[k] : uR[k];
+            const V3<float> nxR = cross(en, ec);
+            float FK[5];
+            flux_adiabatic(u, left ? vL : vR, left ? pL : pR, nxR, Edge<float>{ec, 1.0f, en}, FK);
+            if (side == 0 || side == 3) {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) F[k] = FK[k];
+                return err;
+            }
+            const float S_K = left ? SL : SR;
+            const float iK = __builtin_amdgcn_rcpf(S_K - S_star);
+            const float Dv[5] = {0.0f, -nxR.x, -nxR.y, -nxR.z, S_star};
+#pragma unroll
+            for (int k = 0; k < 5; ++k) F[k] = (S_star * (S_K * u[k] - FK[k]) + S_K * P_LR * Dv[k]) * iK;
+            if constexpr (SOLVER == splash::kHLLCPLUS) {
+                const float dV = VR - VL;
+                const float coef = S_K * iK * g;
+                const float fd = fm1 * dV;
+                const V3<float> vff = v3(en.x * fd + (dvel.x - en.x * dV) * coef, en.y * fd + (dvel.y - en.y * dV) * coef,
+                                         en.z * fd + (dvel.z - en.z * dV) * coef);
+                const V3<float> am = cross(ec, vff);
+                const float pp = phi_L * phi_R * __builtin_amdgcn_rcpf(phi_R - phi_L);
+                F[1] += am.x * pp;
+                F[2] += am.y * pp;
+                F[3] += am.z * pp;
+                F[4] += fm1 * dV * S_star * pp;
+            }
+            return err;
+        }
     }
 }
 
@@ -385,10 +502,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     const bool low = (rec >> 18) & 1u;
     float F[DIM];
     if (t < n_edges) {
-        Edge<float> e;
-        e.ec = v3(e4.x, e4.y, e4.z);
-        e.ecn = e4.w;
-        e.en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
+        const V3<float> ec = v3(e4.x, e4.y, e4.z);
+        const V3<float> en = low ? v3(n4.x, n4.y, n4.z) : v3(-n4.x, -n4.y, -n4.z);
         float uL[DIM], uR[DIM];
         const int sl = low ? sw : tws, sr = low ? tws : sw;
 #pragma unroll
@@ -396,7 +511,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
             uL[k] = sUP[sl * DIM + k];
             uR[k] = sUP[sr * DIM + k];
         }
-        const int err = flux_star<float, DIM, SOLVER>(float(P.gam), uL, uR, e, F);
+        const int err = flux32<DIM, SOLVER>(float(P.gam), uL, uR, ec, en, F);
         if (err) raise_stop(sc, err, hbase + sw);
         const float chord = low ? n4.w : -n4.w;
 #pragma unroll
