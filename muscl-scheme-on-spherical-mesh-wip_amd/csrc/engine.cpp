@@ -132,6 +132,7 @@ void Engine::upload(const RunHost& r) {
     d.acc_rate = d.rd_area = nullptr;
     d.fset_off = d.fset_face = nullptr;
     d.fset_max = 0;
+    d.slot_max = 0;
     args_.W32 = args_.W32b = nullptr;
     if (precision_ == kMixed) {
         const size_t n_ext = L.ext_loc.size() / 4;
@@ -148,6 +149,7 @@ void Engine::upload(const RunHost& r) {
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_off.size(), L.fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_face.size(), L.fset_face.data()));
         d.fset_max = m.fset_max;
+        d.slot_max = m.slot_max;
         const size_t row = dim_ == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));

@@ -84,6 +84,7 @@ struct GeomPtrs {
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
     const double* rd_area;    // [nf] round_diff / area
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
+    int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2] by logical block: max(|v|, c_s), min friction dt
     // ensembles: several runs' patches in one grid (null / 1 for one run)

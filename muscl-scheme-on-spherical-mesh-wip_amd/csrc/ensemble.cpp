@@ -85,7 +85,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     std::vector<float> slot_b, edge32;
     std::vector<double> fcn4, area, rate, rd_area, U;
     std::vector<PhysParams> phys;
-    int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1;
+    int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1, slot_max = 4;
     for (int r = 0; r < R; ++r) {
         const RunHost& rh = hosts[size_t(r)];
         if (rh.dim != dim || rh.solver != solver_) throw std::invalid_argument("ensemble runs differ in dim or solver");
@@ -116,6 +116,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
             for (int c = 0; c < dim; ++c) U.push_back(U_in[size_t(r)][size_t(L.perm[k]) * dim + c]);
         phys.push_back(rh.phys);
         fset_max = std::max(fset_max, m.fset_max);
+        slot_max = std::max(slot_max, m.slot_max);
         faces_.push_back(nf);
         face_off_.push_back(F);
         perm_.push_back(L.perm);
@@ -157,6 +158,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));
         d.fset_max = fset_max;
+        d.slot_max = slot_max;
         d.n_blocks = int32_t(blk_face.size()) - 1;
         d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
         d.blk_run = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_run.size(), blk_run.data()));

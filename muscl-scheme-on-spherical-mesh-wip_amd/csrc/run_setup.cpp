@@ -258,6 +258,10 @@ MixedTables pack_mixed_tables(const RunHost& r) {
         m.rd_area[k] = r.geo.round_diff / r.area[k];
     }
     for (size_t q = 0; q + 1 < L.fset_off.size(); ++q) m.fset_max = std::max(m.fset_max, L.fset_off[q + 1] - L.fset_off[q]);
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t n_slots = L.foff[L.blk_face[b + 1]] - L.foff[L.blk_face[b]] + L.halo_off[b + 1] - L.halo_off[b];
+        m.slot_max = std::max(m.slot_max, (n_slots + 3) / 4 * 4);
+    }
     return m;
 }
 

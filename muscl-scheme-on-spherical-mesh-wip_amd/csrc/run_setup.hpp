@@ -68,6 +68,7 @@ struct MixedTables {
     std::vector<float> slot_b, edge32;
     std::vector<double> rate, rd_area;
     int32_t fset_max = 1;
+    int32_t slot_max = 4;  // largest own + halo slot count of a patch, a multiple of 4
 };
 MixedTables pack_mixed_tables(const RunHost& r);
 

@@ -400,10 +400,16 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
                                                        Scalars* __restrict__ scv, double* __restrict__ U,
                                                        const float* __restrict__ W32,
                                                        float* __restrict__ W32_out) {
-    // reconstructed half-edge states; after the flux phase has read them,
-    // the same rows hold each own slot's flux times chord length
-    __shared__ float sUP[kSlots * DIM];
+    // dynamic LDS: the reconstructed half-edge states [slot_max][DIM] (after
+    // the flux phase has read them, the same rows hold each own slot's flux
+    // times chord length), then the face set's FP32 states (float4 Sigma, L
+    // rows; dim 5: pressures, sound speeds), each sized by the layout
+    extern __shared__ float4 lds_dyn[];
+    float* const sUP = reinterpret_cast<float*>(lds_dyn);
     float* const sF = sUP;
+    float4* const sW = lds_dyn + (g.slot_max * DIM) / 4;     // slot_max is a multiple of 4
+    float* const sP = reinterpret_cast<float*>(sW + g.fset_max);  // dim 5: pressure
+    float* const sC = sP + g.fset_max;                              // dim 5: sound speed
     const int b = xcd_block();
     const int run = g.blk_run ? g.blk_run[b] : 0;  // ensembles: this patch's run
     const PhysParams P = Pv[run];
@@ -423,9 +429,6 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     // phase 0: stage the FP32 states of the block's face set in LDS (own
     // faces first, so face f0 + t sits at local index t); the first slot's
     // rows are requested first so the loads overlap
-    extern __shared__ float4 sW[];
-    float* sP = reinterpret_cast<float*>(sW + g.fset_max);  // dim 5: pressure
-    float* sC = sP + g.fset_max;                              // dim 5: sound speed
     SlotRows r0;
     if (t < n_slots) r0 = load_slot(g, ext0 + uint32_t(t), edge_base);
     {
@@ -606,8 +609,8 @@ namespace {
 
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
-    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 24 : 16);
-    if (a.geom.fset_max > 1024 || lds + sizeof(float) * kSlots * DIM > 64 * 1024)
+    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 24 : 16) + sizeof(float) * DIM * size_t(a.geom.slot_max);
+    if (a.geom.fset_max > 1024 || a.geom.slot_max > kSlots || a.geom.slot_max % 4 || lds > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
         launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
