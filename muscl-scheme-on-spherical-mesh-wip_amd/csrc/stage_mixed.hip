@@ -472,18 +472,14 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     }
     // face data of phase 3, requested here so the flux phase hides the
     // loads without holding them through the reconstruction
-    double u0[DIM], rd_area = 0, rate = 0;
+    double rate = 0;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
     if (face) {
         hb = *row(g.foff, uint32_t(i));
         he = *row(g.foff, uint32_t(i + 1));
-        const double* ui = row(U, uint32_t(i * DIM));
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
         const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
         fcn = v3(c4.x, c4.y, c4.z);
-        rd_area = *row(g.rd_area, uint32_t(i));
         if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
     }
     uint32_t rec = 0;
@@ -534,18 +530,20 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     // combine and density floor for one face per thread
     double vmax = 1e-8, dtx = 1e20;
     if (face) {
+        // U0 and 1/A are first needed after the source and the edge sum,
+        // which hide these loads
+        double u0[DIM];
+        const double* ui = row(U, uint32_t(i * DIM));
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
+        const double rd_area = *row(g.rd_area, uint32_t(i));
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
-        double res[DIM];
+        float src[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) res[k] = 0;
+        for (int k = 0; k < DIM; ++k) src[k] = 0.0f;
         if constexpr (DIM == 5) {
-            if (P.accretion_on) {
-                // on the face's stage input (p-form), from LDS
-                float src[5];
-                source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), float(rate), src);
-#pragma unroll
-                for (int k = 0; k < 5; ++k) res[k] = dt_here * double(src[k]);
-            }
+            // on the face's stage input (p-form), from LDS
+            if (P.accretion_on) source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), float(rate), src);
         }
         // Kahan-compensated FP64 sum of the FP32 edge terms: c carries the
         // (negated) low part each addition loses into the next term.  The
@@ -570,7 +568,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         const double coef = rd_area * dt_here;
         double un[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[k] = (res[k] - coef * (sum[k] - c[k])) + u0[k];
+        for (int k = 0; k < DIM; ++k) un[k] = (dt_here * double(src[k]) - coef * (sum[k] - c[k])) + u0[k];
         if (un[0] < 1e-8) un[0] = 1e-8;
         double wn[DIM];
         face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
