@@ -41,6 +41,13 @@ struct PhysParams {
     double gam_h;       // gam / (gam - 1)
     double gam3d_rho;   // gam3d / (2 gam3d - 1), gam3d = 1 / (2 - gam)
     double cool_k;      // 12/5 k_m (3/4 c_sigma)^(1/4) of the cooling coefficient C
+    // FP32 copies of what the mixed kernel's FP32 arithmetic uses, so its
+    // waves neither divide nor convert the uniform FP64 parameters
+    struct {
+        float gam, inv_gm1, kq;  // gam, 1/(gam-1), (gam+1)/(2 gam)
+        float gam_h, gam3d_rho, cool_k, e_acc, alpha, omega_ns, dmdt_coef;
+        float om_acc[3];
+    } f;
 };
 
 // Device tables in the internal face order (faces sorted along a Morton

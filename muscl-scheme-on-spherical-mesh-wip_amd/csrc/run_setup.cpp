@@ -70,6 +70,17 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
     P.gam_h = gam / (gam - 1);
     P.gam3d_rho = gam3d / (2 * gam3d - 1);
     P.cool_k = 12. / 5 * 1.6e-13 * std::pow(3. / 4 * 4.85e36, 0.25);  // adiabatic.hpp cooling constants
+    P.f.gam = float(gam);
+    P.f.inv_gm1 = float(1 / (gam - 1));
+    P.f.kq = float((gam + 1) / (2 * gam));
+    P.f.gam_h = float(P.gam_h);
+    P.f.gam3d_rho = float(P.gam3d_rho);
+    P.f.cool_k = float(P.cool_k);
+    P.f.e_acc = float(P.e_acc);
+    P.f.alpha = float(P.alpha);
+    P.f.omega_ns = float(P.omega_ns);
+    P.f.dmdt_coef = float(P.dmdt_coef);
+    for (int c = 0; c < 3; ++c) P.f.om_acc[c] = float(P.om_acc[c]);
     r.params = params;
     relayout(r, slots, max_faces);
     return r;

@@ -147,7 +147,7 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
     if constexpr (DIM == 5) {
         // E_fc (MUSCL_base.hpp:541-562): Sigma |v|^2 / 2 = |ec x L|^2 / (2 Sigma)
         const V3<float> c = cross(ec, v3(out[1], out[2], out[3]));
-        out[4] = fmaf(out[4], float(1 / (P.gam - 1)), dot(c, c) * (0.5f * __builtin_amdgcn_rcpf(out[0])));
+        out[4] = fmaf(out[4], P.f.inv_gm1, dot(c, c) * (0.5f * __builtin_amdgcn_rcpf(out[0])));
     }
 }
 
@@ -158,10 +158,10 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
 // drop out), one reciprocal of each side's Sigma, Sigma |v|^2 without the
 // square root, and HLLC+'s velocity-jump factors formed before the star
 // flux so the two side velocities are dead by then.  ec, en: unit edge
-// midpoint and the normal of the left state's face.
+// midpoint and the normal of the left state's face; kq = (gam+1)/(2 gam).
 template <int DIM, int SOLVER>
-__device__ __forceinline__ int flux32(float gam, const float* uL, const float* uR, V3<float> ec, V3<float> en,
-                                      float* F) {
+__device__ __forceinline__ int flux32(float gam, float kq, const float* uL, const float* uR, V3<float> ec,
+                                      V3<float> en, float* F) {
     const float iL = __builtin_amdgcn_rcpf(uL[0]), iR = __builtin_amdgcn_rcpf(uR[0]);
     const V3<float> vL = cross(ec, v3(uL[1], uL[2], uL[3])) * (-iL);  // (ec x L) / -Sigma
     const V3<float> vR = cross(ec, v3(uR[1], uR[2], uR[3])) * (-iR);
@@ -182,7 +182,6 @@ __device__ __forceinline__ int flux32(float gam, const float* uL, const float* u
         const float aL = m_sqrt(gam * pL * iL), aR = m_sqrt(gam * pR * iR);
         // PVRS wave speeds (adiabatic.hpp:594-683)
         const float p_star = smax(0.0f, 0.5f * (pL + pR) - 0.125f * (VR - VL) * (uL[0] + uR[0]) * (aL + aR));
-        const float kq = (gam + 1.0f) * __builtin_amdgcn_rcpf(2.0f * gam);
         float qL = 1.0f, qR = 1.0f;
         if (!(p_star <= pR)) {
             qR = m_sqrt(1.0f + kq * (p_star * __builtin_amdgcn_rcpf(pR) - 1.0f));
@@ -283,19 +282,19 @@ __device__ __forceinline__ void source32(const PhysParams& P, float4 x, float p,
     const float inv0 = __builtin_amdgcn_rcpf(sig), invp = __builtin_amdgcn_rcpf(p);
     const V3<float> L = v3(x.y, x.z, x.w);
     const V3<float> vel = cross(r, L) * (-inv0);
-    const V3<float> omxr = cross(v3(float(P.om_acc[0]), float(P.om_acc[1]), float(P.om_acc[2])), r);
+    const V3<float> omxr = cross(v3(P.f.om_acc[0], P.f.om_acc[1], P.f.om_acc[2]), r);
     const V3<float> rxo = cross(r, omxr);
     res[0] = rate;
     res[1] = rate * rxo.x;
     res[2] = rate * rxo.y;
     res[3] = rate * rxo.z;
     const V3<float> dvv = omxr - vel;
-    res[4] = rate * (float(P.e_acc) + dot(dvv, dvv) * 0.5f);
+    res[4] = rate * (P.f.e_acc + dot(dvv, dvv) * 0.5f);
     const float vn2 = dot(vel, vel);
-    const float h_spec = float(P.gam_h) * p * inv0;  // gam/(gam-1) p/Sigma
+    const float h_spec = P.f.gam_h * p * inv0;  // gam/(gam-1) p/Sigma
     if (!P.friction_on) {
         const V3<float> rxv = cross(r, vel);
-        const float dmdt = float(P.dmdt_coef) * sig;
+        const float dmdt = P.f.dmdt_coef * sig;
         res[0] += dmdt;
         res[1] += dmdt * rxv.x;
         res[2] += dmdt * rxv.y;
@@ -303,7 +302,7 @@ __device__ __forceinline__ void source32(const PhysParams& P, float4 x, float p,
         res[4] += dmdt * (h_spec + vn2 * 0.5f);
     }
     const float g_eff = float(kGM) - vn2;
-    const float C = float(P.cool_k) * sig * invp * m_sqrt(m_sqrt(g_eff * sig));
+    const float C = P.f.cool_k * sig * invp * m_sqrt(m_sqrt(g_eff * sig));
     constexpr float beta_switch = 0.736194670678821f;
     constexpr float C_switch = -1 - 1 / (beta_switch - 1);
     float omb;  // 1 - beta
@@ -317,10 +316,10 @@ __device__ __forceinline__ void source32(const PhysParams& P, float4 x, float p,
     if (omb < 1e-8f) omb = 1e-8f;              // beta above the ceiling
     res[4] -= g_eff * float(1 / 3.4e6) * omb;  // kappa
     if (P.friction_on) {
-        const float rho = float(P.gam3d_rho) * g_eff * sig * sig * invp;
-        const V3<float> vel0 = cross(v3(0.0f, 0.0f, float(P.omega_ns)), r);
+        const float rho = P.f.gam3d_rho * g_eff * sig * sig * invp;
+        const V3<float> vel0 = cross(v3(0.0f, 0.0f, P.f.omega_ns), r);
         const V3<float> dv0 = vel - vel0;
-        const float a = float(P.alpha) * rho * m_sqrt(dot(dv0, dv0));
+        const float a = P.f.alpha * rho * m_sqrt(dot(dv0, dv0));
         const V3<float> wr = dv0 * (-a);
         const V3<float> l_fr = cross(r, wr);
         const float a0 = a * inv0;
@@ -453,7 +452,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
                 if constexpr (DIM == 5) {
                     const float pq = *row(W32, uint32_t(4 * g.nf) + f[j]);
                     sP[q] = pq;
-                    sC[q] = m_sqrt(m_div(float(P.gam) * pq, w.x));
+                    sC[q] = m_sqrt(m_div(P.f.gam * pq, w.x));
                 }
             }
         }
@@ -510,7 +509,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
             uL[k] = sUP[sl * DIM + k];
             uR[k] = sUP[sr * DIM + k];
         }
-        const int err = flux32<DIM, SOLVER>(float(P.gam), uL, uR, ec, en, F);
+        const int err = flux32<DIM, SOLVER>(P.f.gam, P.f.kq, uL, uR, ec, en, F);
         if (err) raise_stop(sc, err, hbase + sw);
         const float chord = low ? n4.w : -n4.w;
 #pragma unroll
