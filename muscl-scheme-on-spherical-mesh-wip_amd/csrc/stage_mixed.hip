@@ -142,13 +142,9 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         const float nu = (1.0f - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
         limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
     }
+    // dim 5: slot 4 stays the limited pressure; flux32 forms E_fc from it
 #pragma unroll
     for (int k = 0; k < DIM; ++k) out[k] = fmaf(dp[k] * hs.y, lim[k], wi[k]);
-    if constexpr (DIM == 5) {
-        // E_fc (MUSCL_base.hpp:541-562): Sigma |v|^2 / 2 = |ec x L|^2 / (2 Sigma)
-        const V3<float> c = cross(ec, v3(out[1], out[2], out[3]));
-        out[4] = fmaf(out[4], P.f.inv_gm1, dot(c, c) * (0.5f * __builtin_amdgcn_rcpf(out[0])));
-    }
 }
 
 // flux_star in FP32 (isothermal.hpp:141-260, HLLE_p.hpp:34-55, HLLC.hpp:
@@ -158,10 +154,15 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
 // drop out), one reciprocal of each side's Sigma, Sigma |v|^2 without the
 // square root, and HLLC+'s velocity-jump factors formed before the star
 // flux so the two side velocities are dead by then.  ec, en: unit edge
-// midpoint and the normal of the left state's face; kq = (gam+1)/(2 gam).
+// midpoint and the normal of the left state's face; kq = (gam+1)/(2 gam),
+// inv_gm1 = 1/(gam-1).  Adiabatic states arrive with slot 4 = the limited
+// pressure p+: the reference converts it to E at the edge midpoint (E_fc,
+// MUSCL_base.hpp:541-562) and pressure() converts back with the same
+// velocity, so p = max(1e-16, p+) and E = p+/(gam-1) + Sigma |v|^2 / 2 are
+// formed here from p+ directly.
 template <int DIM, int SOLVER>
-__device__ __forceinline__ int flux32(float gam, float kq, const float* uL, const float* uR, V3<float> ec,
-                                      V3<float> en, float* F) {
+__device__ __forceinline__ int flux32(float gam, float kq, float inv_gm1, const float* uL, const float* uR,
+                                      V3<float> ec, V3<float> en, float* F) {
     const float iL = __builtin_amdgcn_rcpf(uL[0]), iR = __builtin_amdgcn_rcpf(uR[0]);
     const V3<float> vL = cross(ec, v3(uL[1], uL[2], uL[3])) * (-iL);  // (ec x L) / -Sigma
     const V3<float> vR = cross(ec, v3(uR[1], uR[2], uR[3])) * (-iR);
@@ -177,8 +178,9 @@ __device__ __forceinline__ int flux32(float gam, float kq, const float* uL, cons
         return hlle_combine<float, 4>(SL, SR, uL, uR, FL, FR, F) ? splash::kStopWaveBranch : 0;
     } else {
         int err = 0;
-        const float pL = smax(1e-16f, (uL[4] - uL[0] * dot(vL, vL) * 0.5f) * (gam - 1.0f));
-        const float pR = smax(1e-16f, (uR[4] - uR[0] * dot(vR, vR) * 0.5f) * (gam - 1.0f));
+        const float pL = smax(1e-16f, uL[4]), pR = smax(1e-16f, uR[4]);
+        const float eL[5] = {uL[0], uL[1], uL[2], uL[3], fmaf(uL[4], inv_gm1, uL[0] * dot(vL, vL) * 0.5f)};
+        const float eR[5] = {uR[0], uR[1], uR[2], uR[3], fmaf(uR[4], inv_gm1, uR[0] * dot(vR, vR) * 0.5f)};
         const float aL = m_sqrt(gam * pL * iL), aR = m_sqrt(gam * pR * iR);
         // PVRS wave speeds (adiabatic.hpp:594-683)
         const float p_star = smax(0.0f, 0.5f * (pL + pR) - 0.125f * (VR - VL) * (uL[0] + uR[0]) * (aL + aR));
@@ -196,9 +198,9 @@ __device__ __forceinline__ int flux32(float gam, float kq, const float* uL, cons
         if constexpr (SOLVER == splash::kHLLE_P) {
             const V3<float> nxR = cross(en, ec);
             float FL[5], FR[5];
-            flux_adiabatic(uL, vL, pL, nxR, Edge<float>{ec, 1.0f, en}, FL);
-            flux_adiabatic(uR, vR, pR, nxR, Edge<float>{ec, 1.0f, en}, FR);
-            if (hlle_combine<float, 5>(SL, SR, uL, uR, FL, FR, F)) err |= splash::kStopWaveBranch;
+            flux_adiabatic(eL, vL, pL, nxR, Edge<float>{ec, 1.0f, en}, FL);
+            flux_adiabatic(eR, vR, pR, nxR, Edge<float>{ec, 1.0f, en}, FR);
+            if (hlle_combine<float, 5>(SL, SR, eL, eR, FL, FR, F)) err |= splash::kStopWaveBranch;
             return err;
         } else {
             const float phi_L = uL[0] * (SL - VL), phi_R = uR[0] * (SR - VR);
@@ -234,7 +236,7 @@ __device__ __forceinline__ int flux32(float gam, float kq, const float* uL, cons
             const bool left = side <= 1;
             float u[5];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) u[k] = left ? uL[k] : uR[k];
+            for (int k = 0; k < 5; ++k) u[k] = left ? eL[k] : eR[k];
             const V3<float> nxR = cross(en, ec);
             float FK[5];
             flux_adiabatic(u, left ? vL : vR, left ? pL : pR, nxR, Edge<float>{ec, 1.0f, en}, FK);
@@ -345,22 +347,23 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
     const double vv2 = dot(rl, rl) * (inv * inv);
 #pragma unroll
     for (int k = 0; k < DIM; ++k) w[k] = u[k];
-    double cs;
+    double cs2;  // c_s^2
     if constexpr (DIM == 5) {
         const double p = smax(1e-16, (u[4] - u[0] * vv2 * 0.5) * (P.gam - 1));
         w[4] = p;
-        cs = sqrt(P.gam * p * inv);
+        cs2 = P.gam * p * inv;
         const double g_eff = kGM - vv2;
         const V3<double> vel = rl * (-inv);
         const V3<double> vel0 = cross(v3(0.0, 0.0, P.omega_ns), fcn);
         const double cand = 0.01 * p / (P.alpha * P.gam3d_rho * g_eff * u[0] * norm(vel - vel0));
         if (dtx > cand) dtx = cand;
     } else {
-        cs = 1.0;
+        cs2 = 1.0;
     }
-    const double vn = sqrt(vv2);
-    if (vn > vmax) vmax = vn;
-    if (cs > vmax) vmax = cs;
+    // max(|v|, c_s) as one square root (sqrt is monotonic; fmax skips a NaN
+    // operand, as the two comparisons of find_v_max do)
+    const double vc = sqrt(__builtin_fmax(vv2, cs2));
+    if (vc > vmax) vmax = vc;
 }
 
 // One slot's rows: packed record (x, y, beta+0, beta-0), {H+/H-, BM/H+,
@@ -509,7 +512,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
             uL[k] = sUP[sl * DIM + k];
             uR[k] = sUP[sr * DIM + k];
         }
-        const int err = flux32<DIM, SOLVER>(P.f.gam, P.f.kq, uL, uR, ec, en, F);
+        const int err = flux32<DIM, SOLVER>(P.f.gam, P.f.kq, P.f.inv_gm1, uL, uR, ec, en, F);
         if (err) raise_stop(sc, err, hbase + sw);
         const float chord = low ? n4.w : -n4.w;
 #pragma unroll
