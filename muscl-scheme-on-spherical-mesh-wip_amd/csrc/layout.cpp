@@ -1,6 +1,7 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 
@@ -8,15 +9,36 @@ namespace splash {
 
 namespace {
 
-// spread the low 21 bits of x to every third bit (3-D Morton interleave)
-uint64_t spread3(uint64_t x) {
-    x &= 0x1fffff;
-    x = (x | x << 32) & 0x1f00000000ffffull;
-    x = (x | x << 16) & 0x1f0000ff0000ffull;
-    x = (x | x << 8) & 0x100f00f00f00f00full;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
+// spread the low 24 bits of x to every second bit (2-D Morton interleave)
+uint64_t spread2(uint64_t x) {
+    x &= 0xffffff;
+    x = (x | x << 16) & 0x0000ffff0000ffffull;
+    x = (x | x << 8) & 0x00ff00ff00ff00ffull;
+    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | x << 2) & 0x3333333333333333ull;
+    x = (x | x << 1) & 0x5555555555555555ull;
     return x;
+}
+
+// Cube-map Morton code of a unit face centre: the cube face it projects to
+// (dominant axis and sign) in the top bits, then the 2-D Morton interleave
+// of its gnomonic coordinates on that face (the other two components over
+// the dominant one, in [-1, 1], at 24 bits).  Runs of this order are compact
+// on the sphere, so a patch of consecutive faces has a shorter boundary
+// (fewer halo half-edges) than under a 3-D Morton curve, whose cubes the
+// sphere cuts at arbitrary angles.
+uint64_t cube_morton(const double* c) {
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+        if (std::abs(c[k]) > std::abs(c[ax])) ax = k;
+    const uint64_t face = uint64_t(2 * ax + (c[ax] > 0 ? 1 : 0));
+    const double d = std::abs(c[ax]);
+    const double top = double((1u << 24) - 1);
+    auto q = [&](double x) {
+        const double v = (x / d + 1.0) * 0.5 * top;
+        return uint64_t(std::min(std::max(v, 0.0), top));
+    };
+    return face << 48 | spread2(q(c[(ax + 1) % 3])) | spread2(q(c[(ax + 2) % 3])) << 1;
 }
 
 void require(bool ok, const char* what) {
@@ -31,15 +53,7 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     L.slots = slots;
     L.max_faces = max_faces;
     std::vector<uint64_t> code(nf);
-    for (int32_t f = 0; f < nf; ++f) {
-        uint64_t c = 0;
-        for (int k = 0; k < 3; ++k) {
-            const double v = (g.fcn[3 * size_t(f) + k] + 1.0) * 0.5 * double((1u << 21) - 1);
-            const uint64_t q = uint64_t(std::min(std::max(v, 0.0), double((1u << 21) - 1)));
-            c |= spread3(q) << k;
-        }
-        code[f] = c;
-    }
+    for (int32_t f = 0; f < nf; ++f) code[f] = cube_morton(&g.fcn[3 * size_t(f)]);
     L.perm.resize(nf);
     for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
     std::stable_sort(L.perm.begin(), L.perm.end(), [&](int32_t x, int32_t y) { return code[x] < code[y]; });
