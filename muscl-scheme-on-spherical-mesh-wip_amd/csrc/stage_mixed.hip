@@ -19,22 +19,20 @@ __device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_
 __device__ __forceinline__ float m_pow(float x, float y) {
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));  // x > 0 at every call site
 }
-__device__ __forceinline__ float m_tanh(float x) {
-    // tanh|x| = 1 - 2 / (e^{2|x|} + 1), sign restored; saturates to 1 for large |x|
-    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * fabsf(x));  // 2/ln 2
-    return copysignf(1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f), x);
-}
 
 #include "stage_kernels.inc"
 
 // ----------------------------------------------------------------------
-// The mixed-precision stage kernel.  It follows k_stage (same patches, LDS
-// slots and face loop) but works on FP32 copies: the stage input as one
-// float4 (Sigma, L) row per face plus (dim 5) one pressure per face, staged
-// per block in LDS, and four float4 rows plus one packed 8-byte record per
-// slot.  The state, the RK combine, the dt limits and reductions and the
-// flux accumulation (Kahan-compensated) stay FP64; the accretion /
-// cooling / friction sources run in FP32 on the FP32 stage input.
+// The mixed-precision stage kernel.  It follows k_stage (patches of faces,
+// LDS slots for their half-edges and halo, one face loop) but works on FP32
+// copies: the stage input as one float4 (Sigma, L) row per face plus (dim 5)
+// one pressure per face, staged per block in LDS; per slot a 16-byte packed
+// record (stencil face-set indices, owner, edge, valence, beta) and a
+// float4 of H ratios; per edge a float4 {ec, |ec|}, a float4 {en, chord}
+// and a 4-byte slot record.  The state, the RK combine, the dt limits and
+// reductions and the flux accumulation (Kahan-compensated) stay FP64; the
+// accretion / cooling / friction sources run in FP32 on the FP32 stage
+// input.
 
 // Patches: at most kFaces faces (one wave updates them) whose own
 // half-edges plus halo fit kSlots LDS slots, two slots per thread.
@@ -108,14 +106,15 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
 // block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed): li
 // = owner, plus0, plus1, minus0, minus1 local indices; bp, bm = beta+0,
 // beta-0 (the pair's second weight is 1 - the first); hs = {H+/H-, BM/H+,
-// H+/BM, arc/area}; e4 = {ec, |ec|}; en = the half-edge's outward normal.
+// H+/BM, arc/area}; ec = the unit edge midpoint; en = the half-edge's
+// outward normal.
 // The slopes stay raw differences d+ = u+ - w, d- = w - u-: the reference's
 // r = (d-/H-) / (d+/H+) is (H+/H-) d-/d+ and its U+ = w + (d+/H+) lim BM is
 // w + d+ (BM/H+) lim.
 template <int DIM>
-__device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, const float4* sW, const float* sP,
-                                                const float* sC, const int* li, int valence, float bp, float bm,
-                                                float4 hs, float4 e4, V3<float> en, float* out) {
+__device__ __forceinline__ void reconstruct_lds(float dt, const float4* sW, const float* sP, const float* sC,
+                                                const int* li, int valence, float bp, float bm, float4 hs,
+                                                V3<float> ec, V3<float> en, float* out) {
     float w[5][DIM];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -131,7 +130,6 @@ __device__ __forceinline__ void reconstruct_lds(const PhysParams& P, float dt, c
         const float dm = wi[k] - fmaf(bm, w[3][k] - w[4][k], w[4][k]);
         D[k] = dm * __builtin_amdgcn_rcpf(dp[k]);
     }
-    const V3<float> ec = v3(e4.x, e4.y, e4.z);
     // v . en with v = (ec x L) / (-Sigma |ec|) (dim 5; |ec| is 1 in FP32) or
     // (ec x L) / -Sigma
     const float cl = dot(cross(ec, v3(wi[1], wi[2], wi[3])), en);
@@ -387,14 +385,14 @@ __device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, u
 // minus0 << 20 | negate-en << 30; y = minus1 | owner << 10 | edge << 20 |
 // valence << 28.
 template <int DIM>
-__device__ __forceinline__ void reconstruct_slot(const PhysParams& P, float dt, const float4* sW, const float* sP,
-                                                 const float* sC, const SlotRows& r, float* out) {
+__device__ __forceinline__ void reconstruct_slot(float dt, const float4* sW, const float* sP, const float* sC,
+                                                 const SlotRows& r, float* out) {
     const uint4 ra = r.ra;
     const int li[5] = {int((ra.y >> 10) & 1023u), int(ra.x & 1023u), int((ra.x >> 10) & 1023u),
                        int((ra.x >> 20) & 1023u), int(ra.y & 1023u)};
     const float sg = (ra.x >> 30) & 1u ? -1.0f : 1.0f;
-    reconstruct_lds<DIM>(P, dt, sW, sP, sC, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w),
-                         r.hs, r.e4, v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
+    reconstruct_lds<DIM>(dt, sW, sP, sC, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w), r.hs,
+                         v3(r.e4.x, r.e4.y, r.e4.z), v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
 }
 
 template <int DIM, int SOLVER, int STAGE>
@@ -469,8 +467,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         const int t1 = t + kBlock;
         SlotRows r1;
         if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
-        if (t < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r0, &sUP[t * DIM]);
-        if (t1 < n_slots) reconstruct_slot<DIM>(P, float(dt), sW, sP, sC, r1, &sUP[t1 * DIM]);
+        if (t < n_slots) reconstruct_slot<DIM>(float(dt), sW, sP, sC, r0, &sUP[t * DIM]);
+        if (t1 < n_slots) reconstruct_slot<DIM>(float(dt), sW, sP, sC, r1, &sUP[t1 * DIM]);
     }
     // face data of phase 3, requested here so the flux phase hides the
     // loads without holding them through the reconstruction
