@@ -156,6 +156,12 @@ void Engine::upload(const RunHost& r) {
     }
     d.n_blocks = L.n_blocks();
     d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
+    {
+        const std::vector<double> p2(2 * size_t(fp64::finalize_blocks()), 0.0);
+        const unsigned zero = 0;
+        d.dt_partial2 = static_cast<double*>(dev(sizeof(double) * p2.size(), p2.data()));
+        d.dt_ticket = static_cast<unsigned*>(dev(sizeof(unsigned), &zero));
+    }
     d.blk_run = d.run_blk = nullptr;
     d.n_runs = 1;
 

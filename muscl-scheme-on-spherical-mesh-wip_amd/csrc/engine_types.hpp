@@ -94,6 +94,8 @@ struct GeomPtrs {
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
     double* dt_partial;       // [n_blocks][2] by logical block: max(|v|, c_s), min friction dt
+    double* dt_partial2;      // [n_runs][16][2]: the dt kernel's per-workgroup partials
+    unsigned* dt_ticket;      // [n_runs]: the dt kernel's arrival counters (0 between launches)
     // ensembles: several runs' patches in one grid (null / 1 for one run)
     const int32_t* blk_run;   // [n_blocks] run of each logical block
     const int32_t* run_blk;   // [n_runs+1] block range of each run

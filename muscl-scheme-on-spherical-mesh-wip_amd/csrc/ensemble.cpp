@@ -161,6 +161,10 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.slot_max = slot_max;
         d.n_blocks = int32_t(blk_face.size()) - 1;
         d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
+        const std::vector<double> p2(2 * size_t(fp64::finalize_blocks()) * size_t(R), 0.0);
+        const std::vector<unsigned> tickets(size_t(R), 0u);
+        d.dt_partial2 = static_cast<double*>(dev(sizeof(double) * p2.size(), p2.data()));
+        d.dt_ticket = static_cast<unsigned*>(dev(sizeof(unsigned) * tickets.size(), tickets.data()));
         d.blk_run = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_run.size(), blk_run.data()));
         d.run_blk = static_cast<const int32_t*>(dev(sizeof(int32_t) * run_blk.size(), run_blk.data()));
         d.n_runs = R;

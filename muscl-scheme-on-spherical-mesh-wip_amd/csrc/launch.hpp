@@ -39,6 +39,7 @@ void enqueue_light(const StepArgs& a, const double* weights, double* partial, hi
 // ev = v_j - v_j1 per half-edge [nh][3].
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s);
 int block_size();
+int finalize_blocks();  // dt-kernel workgroups per run
 }  // namespace fp64
 
 namespace mixed {

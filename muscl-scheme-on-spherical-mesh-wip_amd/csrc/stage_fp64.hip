@@ -42,8 +42,10 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s) {
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev) {
     const int runs = a.geom.n_runs;
     if (a.phys.dim == 5)
-        launch(k_finalize_dt<5>, runs, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0, a.dt0_dev);
-    else launch(k_finalize_dt<4>, runs, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0, a.dt0_dev);
+        launch(k_finalize_dt<5>, runs * kFinalizeBlocks, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0,
+               a.dt0_dev);
+    else launch(k_finalize_dt<4>, runs * kFinalizeBlocks, kFinalizeThreads, 0, s, ev, a.geom, a.phys_dev, a.sc, dt0,
+                a.dt0_dev);
 }
 
 
@@ -75,6 +77,7 @@ void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_t
 }
 
 int block_size() { return kBlock; }
+int finalize_blocks() { return kFinalizeBlocks; }
 
 }  // namespace fp64
 }  // namespace splash
