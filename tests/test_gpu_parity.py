@@ -190,6 +190,36 @@ def test_c5b_full_size_mixed_vs_fp64_and_conservation():
         assert abs(d[0] / d0[0] - 1) < 1e-12
 
 
+def test_hex9_fp64_parity_with_split_dt_reduction():
+    """icosphere_hex(9): about 61k FP64 patches, so the dt kernel splits each
+    step's partials over 15 workgroups and the last one finishes dt.  With
+    dt0 = 1e-3 the CFL clamp is active, so dt is exactly that reduction's
+    result: after 2 steps the FP64 state matches the oracle to 1e-10 and dt
+    and time to round-off; the FP32-flux path's first dt (from its own
+    prepare kernel's partials) agrees too."""
+    import splash_amd as S
+    mesh = S.icosphere_hex(9)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=8, omega=(0.1, 0.0, 0.3))
+    p = S.default_params(setup_threads=16)
+    gpu = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, params=p, precision=S.FP64)
+    mixed = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, params=p, precision=S.MIXED)
+    ref = O.Solver(O.Mesh("hex", 9), U0, 5, O.HLLCPLUS, GAM, 0.0, _oparams(p), threads=16)
+    ref.do_step(1e-3)
+    gpu.do_step(1e-3)
+    mixed.do_step(1e-3)
+    dt1 = ref.last_dt()
+    assert dt1 < 1e-3  # the CFL / friction clamp decided dt
+    assert gpu.last_dt() == pytest.approx(dt1, rel=1e-14)
+    assert mixed.last_dt() == pytest.approx(dt1, rel=1e-12)
+    ref.do_step(1e-3)
+    gpu.do_step(1e-3)
+    err = _rel(gpu.get_state(), ref.state())
+    print(f"hex(9) FP64 2 steps: max rel err {err:.3e}, dt {dt1:.6e} / {ref.last_dt():.6e}")
+    assert err <= TOL
+    assert gpu.last_dt() == pytest.approx(ref.last_dt(), rel=1e-14)
+    assert gpu.time() == pytest.approx(ref.time(), rel=1e-14)
+
+
 def test_profile_step_times_back_to_back_steps():
     """profile_step advances exactly n steps, gives positive per-kernel means
     that add up to the step time, and leaves the state where as many plain
