@@ -76,7 +76,12 @@ __device__ __forceinline__ float h_switch32(float r) {
 
 // The limiters of adiabatic.hpp:685-828 and isothermal.hpp:269-320 on
 // r = rho * D (rho = H+/H-, D = (w - u-) / (u+ - w)), with e = H+/BM =
-// eta_plus: eta_minus * r = e * D, and eta_minus = e / rho.
+// eta_plus: eta_minus * r = e * D, and eta_minus = e / rho.  The reference
+// sets the limiter of a NaN r to 0.  Here D = d- * rcp(d+) is NaN only when
+// d+ = d- = 0, or when the state already holds NaN or inf: with a NaN D the
+// hardware min / max give a finite limiter and h(NaN) = 0, and the slope
+// d+ = 0 makes U+ = w exactly as a zero limiter does (a non-finite state
+// stays non-finite either way), so the override needs no instructions.
 template <int DIM>
 __device__ __forceinline__ void limiter32(const float* D, float rho, float e, float nu, int valence, float* lim) {
     if constexpr (DIM == 5) {
@@ -87,8 +92,7 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
             const float rk = rho * D[k];
             const float sb = lmax(0.0f, lmax(lmin(1.0f, D[k] * ed), lmin(rk, e)));
             const float to = lmax(0.0f, lmin(sb, fmaf(c3, rk - 1.0f, 1.0f)));
-            const float l = fmaf(h_switch32(rk), sb - to, to);  // (1-h) third_order + h superbee
-            lim[k] = __builtin_isnan(rk) ? 0.0f : l;
+            lim[k] = fmaf(h_switch32(rk), sb - to, to);  // (1-h) third_order + h superbee
         }
     } else {
         const float ed = e * __builtin_amdgcn_rcpf(float(valence) * nu);  // 2 eta r / den = ed D
@@ -96,8 +100,7 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
             const float rk = rho * D[k];
-            const float l = lmax(0.0f, lmax(lmin(1.0f, D[k] * ed), lmin(rk, eta)));
-            lim[k] = __builtin_isnan(rk) ? 0.0f : l;
+            lim[k] = lmax(0.0f, lmax(lmin(1.0f, D[k] * ed), lmin(rk, eta)));
         }
     }
 }
