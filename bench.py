@@ -16,8 +16,10 @@ Both precision paths run the same steps from the same state:
   fp64        = face-updates/s of the plain FP64 path,
   l1_err      = relative L1 density error of the FP32-flux path vs FP64.
 The roofline object covers the dominant kernel of the FP32-flux path, timed
-with HIP events on the run's stream; the CPU baseline is the oracle (the C
-restatement of the reference step) on a bounded sample of the same workload.
+with HIP events on the run's stream, against the 8 TB/s peak and against a
+device-to-device copy timed on the same GPU (copy_GBs); the CPU baseline is
+the oracle (the C restatement of the reference step) on a bounded sample of
+the same workload.
 """
 import argparse
 import json
@@ -194,6 +196,45 @@ def cpu_baseline(S, wl, U0, dt0, steps, threads):
     return m.nf * steps / el, el
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """Achievable HBM bandwidth on this GPU: a device-to-device hipMemcpy of
+    `mib` MiB (read + write bytes per second), timed with HIP events over
+    `reps` copies after one warm-up copy, through the HIP runtime the solver
+    library itself loaded."""
+    import ctypes as C
+    with open("/proc/self/maps") as maps:  # the runtime libsplash_amd.so is linked against
+        path = next((ln.split()[-1] for ln in maps if "libamdhip64" in ln), "libamdhip64.so.7")
+    hip = C.CDLL(path)
+    nbytes = mib * (1 << 20)
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    src, dst = C.c_void_p(), C.c_void_p()
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMemset(src, 0, C.c_size_t(nbytes)), "hipMemset")
+        ok(hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3), "hipMemcpy")  # 3 = device to device
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +296,7 @@ def main():
     name = KERNELS[k]
     ab = algorithmic_bytes(name, F, H)
     achieved = ab / (prof[k] * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
     step_bytes = 2 * (104 * H + 36 * F) + 40 * 5 * F
     traffic = MEASURED_TRAFFIC.get((args.workload, name))
 
@@ -292,7 +334,7 @@ def main():
         "step_achieved_GBs": step_bytes / (prof[3] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": ab},
+                     "algorithmic_bytes": ab, "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "stop_check": bool(stop),
         "sim_time": [t64, tmx],
     }
