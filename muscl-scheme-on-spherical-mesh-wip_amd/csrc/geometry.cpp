@@ -78,17 +78,41 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
             for (int32_t k = mesh.foff[f]; k < mesh.foff[f + 1]; ++k) vf[fill[FV[k]]++] = f;
     }
 
-    // vertex neighbours (sorted, unique, without the face itself)
-    std::vector<std::vector<int32_t>> nbs(nf);
-#pragma omp parallel for num_threads(threads) schedule(static)
-    for (int32_t f = 0; f < nf; ++f) {
-        auto& list = nbs[f];
+    // vertex neighbours (sorted, unique, without the face itself) as one
+    // CSR array: each thread gathers a face's list into its own scratch
+    // buffer, once to count and once to fill, so the loops allocate nothing
+    // per face
+    auto gather_nbs = [&](int32_t f, std::vector<int32_t>& list) {
+        list.clear();
         for (int32_t k = mesh.foff[f]; k < mesh.foff[f + 1]; ++k)
             for (int32_t q = vf_off[FV[k]]; q < vf_off[FV[k] + 1]; ++q) list.push_back(vf[q]);
         std::sort(list.begin(), list.end());
         list.erase(std::unique(list.begin(), list.end()), list.end());
         list.erase(std::remove(list.begin(), list.end(), f), list.end());
+    };
+    std::vector<int32_t> nb_off(nf + 1, 0), nb_list;
+#pragma omp parallel num_threads(threads)
+    {
+        std::vector<int32_t> list;
+        list.reserve(64);
+#pragma omp for schedule(static)
+        for (int32_t f = 0; f < nf; ++f) {
+            gather_nbs(f, list);
+            nb_off[f + 1] = int32_t(list.size());
+        }
+#pragma omp single
+        {
+            for (int32_t f = 0; f < nf; ++f) nb_off[f + 1] += nb_off[f];
+            nb_list.resize(size_t(nb_off[nf]));
+        }
+#pragma omp for schedule(static)
+        for (int32_t f = 0; f < nf; ++f) {
+            gather_nbs(f, list);
+            std::copy(list.begin(), list.end(), nb_list.begin() + nb_off[f]);
+        }
     }
+    auto nbs_begin = [&](int32_t f) { return nb_list.data() + nb_off[f]; };
+    auto nbs_end = [&](int32_t f) { return nb_list.data() + nb_off[f + 1]; };
 
     g.he_face.resize(nh);
     g.nb_edge.resize(nh);
@@ -101,7 +125,8 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
         for (int32_t j = 0; j < n; ++j) {
             const int32_t a = FV[mesh.foff[f] + j], b = FV[mesh.foff[f] + (j + 1 == n ? 0 : j + 1)];
             int32_t found = -1, count = 0;
-            for (int32_t o : nbs[f]) {
+            for (const int32_t* po = nbs_begin(f); po != nbs_end(f); ++po) {
+                const int32_t o = *po;
                 const auto first = FV.begin() + mesh.foff[o], last = FV.begin() + mesh.foff[o + 1];
                 if (std::find(first, last, a) != last && std::find(first, last, b) != last) {
                     if (found < 0) found = o;
@@ -115,6 +140,8 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
     }
     if (bad) throw std::runtime_error("mesh is not a closed manifold: " + std::to_string(bad) + " edges without a unique neighbour");
     // MUSCL_base.hpp:383-384 reverse slot j0 (first match in neighbors_edge[nb])
+    int asym = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : asym)
     for (int32_t f = 0; f < nf; ++f)
         for (int32_t h = mesh.foff[f]; h < mesh.foff[f + 1]; ++h) {
             const int32_t o = g.nb_edge[h];
@@ -124,9 +151,10 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
                     j0 = q;
                     break;
                 }
-            if (j0 < 0) throw std::runtime_error("edge-neighbour relation is not symmetric");
+            if (j0 < 0) ++asym;
             g.twin[h] = j0;
         }
+    if (asym) throw std::runtime_error("edge-neighbour relation is not symmetric");
 
     g.fc.resize(3 * size_t(nf));
     g.normal.resize(3 * size_t(nf));
@@ -213,7 +241,8 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
             double best_l = -1, best_r = -1;
             int32_t l1 = -1, rr1 = -1;
             const double nBM = norm(BM);
-            for (int32_t o : nbs[f]) {
+            for (const int32_t* po = nbs_begin(f); po != nbs_end(f); ++po) {
+                const int32_t o = *po;
                 const V3 fo = ld(&g.fc[3 * size_t(o)]);
                 V3 d = c - fo;
                 V3 dp = d - nrm * dot(d, nrm);
@@ -242,7 +271,8 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
             const V3 B_right = ld(&g.fc[3 * size_t(rr1)]) - c;
             const V3 B_right_face = B_right - nrm * dot(B_right, nrm);
             const V3 side = cross(c, r1 / 2);
-            for (int32_t o : nbs[f]) {
+            for (const int32_t* po = nbs_begin(f); po != nbs_end(f); ++po) {
+                const int32_t o = *po;
                 const V3 fo = ld(&g.fc[3 * size_t(o)]);
                 V3 d = c - fo;
                 V3 dp = d - nrm * dot(d, nrm);

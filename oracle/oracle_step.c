@@ -708,7 +708,7 @@ void or_curl(const or_solver *s, double *out)
         double vort = 0;
         for (int j = 0; j < n; j++) {
             const double *up = s->Uplus + (size_t)(m->foff[f] + j) * s->dim;
-            v3 l = {up[1], up[2], up[3]};
+            v3 l = v3m(up[1], up[2], up[3]);
             const int j1 = (j == n - 1) ? 0 : j + 1;
             v3 ec = edge_center(s, f, j);
             v3 vel = v3divs(v3cross(ec, l), -s->U[(size_t)f * s->dim]);
