@@ -99,3 +99,30 @@ def test_cpp_ensemble_wrapper_matches_single_solvers(tmp_path):
     r = subprocess.run([prog], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+GOLDEN_PARAMS = os.path.join(os.path.dirname(__file__), "golden", "parameters.json")
+PHYSICS_KEYS = {"accretion_rate": "acc_rate", "e_acc": "e_acc", "omega_acc_abs": "omega_acc_abs",
+                "tilt_angle": "tilt_angle", "acc_width": "acc_width", "alpha": "alpha",
+                "friction_on": "friction_on"}
+
+
+def test_reference_parameter_file_is_the_default_physics():
+    """tests/golden/parameters.json is the reference's shipped input file.
+    The drop-in reader parses it to exactly its values (the keys of
+    adiabatic.hpp:59-68), and those are the library's and the oracle's
+    default physics parameters."""
+    import splash_amd as S
+    with open(GOLDEN_PARAMS) as f:
+        ref = json.load(f)
+    reader = os.path.join(os.path.dirname(__file__), "cpp", "params_reader")
+    r = subprocess.run([reader, GOLDEN_PARAMS], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    parsed = {k: float(v) for k, v in (ln.split() for ln in r.stdout.splitlines())}
+    d = S.default_params()
+    o = O.make_params()
+    for key, field in PHYSICS_KEYS.items():
+        want = float(ref[key])
+        assert parsed[key] == want, key
+        assert float(getattr(d, field)) == want, key
+        assert float(getattr(o, field)) == want, key
