@@ -221,7 +221,7 @@ int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);
 int splash_ensemble_get_state(splash_ensemble *e, int run, double *U_out);
 
 /* Host-side check of the internal GPU layout of a mesh that the
- * mixed-precision kernel uses (cube-map Morton face order, patches of at most 64
+ * mixed-precision kernel uses (coordinate-bisection face order, patches of at most 64
  * faces whose half-edges plus halo fit 512 LDS slots, LDS face sets):
  * builds and validates it and returns {blocks (multiple of 8), non-empty
  * blocks, halo half-edges, max halo, max face set}. */

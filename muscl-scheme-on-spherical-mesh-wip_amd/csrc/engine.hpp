@@ -87,7 +87,7 @@ private:
     int32_t patch_faces_ = 0;
     int32_t nf_ = 0, nh_ = 0;
 
-    // internal layout: faces along a cube-map Morton curve, half-edges face-major
+    // internal layout: faces in bisection or curve order, half-edges face-major
     std::vector<int32_t> perm_;              // internal -> mesh face
     std::vector<int32_t> he_old_;            // internal -> mesh half-edge
 

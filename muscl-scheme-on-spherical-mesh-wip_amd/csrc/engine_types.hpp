@@ -50,8 +50,9 @@ struct PhysParams {
     } f;
 };
 
-// Device tables in the internal face order (faces sorted along a cube-map
-// Morton curve of their centres; half-edges face-major in that order).  Every
+// Device tables in the internal face order (faces ordered by a coordinate
+// bisection or a cube-map Morton curve of their centres; half-edges
+// face-major in that order).  Every
 // per-half-edge value is the one of the reference's [face][edge] table for
 // the same half-edge.
 struct GeomPtrs {

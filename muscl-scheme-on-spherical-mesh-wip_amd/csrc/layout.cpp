@@ -24,9 +24,7 @@ uint64_t spread2(uint64_t x) {
 // (dominant axis and sign) in the top bits, then the 2-D Morton interleave
 // of its gnomonic coordinates on that face (the other two components over
 // the dominant one, in [-1, 1], at 24 bits).  Runs of this order are compact
-// on the sphere, so a patch of consecutive faces has a shorter boundary
-// (fewer halo half-edges) than under a 3-D Morton curve, whose cubes the
-// sphere cuts at arbitrary angles.
+// on the sphere, and a patch may take any number of consecutive faces.
 uint64_t cube_morton(const double* c) {
     int ax = 0;
     for (int k = 1; k < 3; ++k)
@@ -41,6 +39,82 @@ uint64_t cube_morton(const double* c) {
     return face << 48 | spread2(q(c[(ax + 1) % 3])) | spread2(q(c[(ax + 2) % 3])) << 1;
 }
 
+// Recursive coordinate bisection of the unit face centres: the range
+// p[lo, hi) is split along the coordinate of its largest extent, at a
+// multiple of `leaf` near its middle, until each range holds at most `leaf`
+// faces.  Every range starts at a multiple of `leaf`, so every leaf but the
+// last is full.  A leaf is a box of bounded aspect on the sphere, whose
+// boundary (a patch's halo half-edges) is shorter than that of a run of a
+// space-filling curve, which follows the curve's turns: about a fifth fewer
+// halo half-edges at 64 faces.  The split orders by (coordinate, face index),
+// a strict total order, and each leaf is sorted by face index, so the
+// permutation does not depend on the standard library's algorithms.
+void bisect(std::vector<int32_t>& p, size_t lo, size_t hi, const std::vector<double>& c, size_t leaf) {
+    const size_t m = hi - lo;
+    if (m <= leaf) {
+        std::sort(p.begin() + lo, p.begin() + hi);
+        return;
+    }
+    double mn[3] = {c[3 * size_t(p[lo])], c[3 * size_t(p[lo]) + 1], c[3 * size_t(p[lo]) + 2]};
+    double mx[3] = {mn[0], mn[1], mn[2]};
+    for (size_t k = lo + 1; k < hi; ++k)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], c[3 * size_t(p[k]) + a]);
+            mx[a] = std::max(mx[a], c[3 * size_t(p[k]) + a]);
+        }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a)
+        if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+    // leaf <= half <= m/2 + leaf/2 < m
+    const size_t half = leaf * std::max<size_t>(1, (m + leaf) / (2 * leaf));
+    std::nth_element(p.begin() + lo, p.begin() + lo + half, p.begin() + hi, [&](int32_t x, int32_t y) {
+        const double cx = c[3 * size_t(x) + ax], cy = c[3 * size_t(y) + ax];
+        return cx < cy || (cx == cy && x < y);
+    });
+    bisect(p, lo, lo + half, c, leaf);
+    bisect(p, lo + half, hi, c, leaf);
+}
+
+
+// Greedy patches over the face order perm (iperm its inverse): add faces
+// while own half-edges plus halo fit the slots and the face count stays
+// within max_faces; with leaf > 0 a patch also never spans two runs of
+// `leaf` faces (a bisection leaf whose halo overflows the slots is split).
+// Returns the patches' first faces and the end, nf.
+std::vector<int32_t> pack(const Geometry& g, const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
+                          int32_t slots, int32_t max_faces, int32_t leaf) {
+    const int32_t nf = g.nf;
+    std::vector<int32_t> blk{0};
+    int32_t own = 0, halo = 0;
+    for (int32_t k = 0; k < nf; ++k) {
+        const int32_t f = perm[k];
+        const int32_t val = g.foff[f + 1] - g.foff[f];
+        if (val > slots) throw std::runtime_error("face valence exceeds the block's LDS slots");
+        int32_t dh = 0;
+        const int32_t f0 = blk.back();
+        for (int32_t h = g.foff[f]; h < g.foff[f + 1]; ++h) {
+            const int32_t o = iperm[g.he_face[g.twin[h]]];
+            dh += (o >= f0 && o < k) ? -1 : 1;
+        }
+        if (own + val + halo + dh > slots || k - f0 >= max_faces || (leaf > 0 && k > f0 && k % leaf == 0)) {
+            blk.push_back(k);
+            own = 0;
+            halo = 0;
+            dh = val;
+        }
+        own += val;
+        halo += dh;
+    }
+    blk.push_back(nf);
+    return blk;
+}
+
+std::vector<int32_t> inverse(const std::vector<int32_t>& perm) {
+    std::vector<int32_t> inv(perm.size());
+    for (size_t k = 0; k < perm.size(); ++k) inv[size_t(perm[k])] = int32_t(k);
+    return inv;
+}
+
 void require(bool ok, const char* what) {
     if (!ok) throw std::logic_error(std::string("layout check failed: ") + what);
 }
@@ -52,13 +126,36 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     Layout L;
     L.slots = slots;
     L.max_faces = max_faces;
-    std::vector<uint64_t> code(nf);
-    for (int32_t f = 0; f < nf; ++f) code[f] = cube_morton(&g.fcn[3 * size_t(f)]);
+    if (max_faces < 1) throw std::runtime_error("patch face cap must be positive");
+    for (size_t k = 0; k < 3 * size_t(nf); ++k)
+        if (!std::isfinite(g.fcn[k])) throw std::runtime_error("a face centre is not finite (degenerate face)");
+    // Two face orders; the layout keeps the one with fewer patches, the
+    // bisection on a tie.  Bisection leaves of max_faces faces have the
+    // shortest boundaries and reach the fewest patches, ceil(nf / max_faces),
+    // when max_faces faces and their halo fit the slots (the FP32-flux
+    // layout).  Where the slots bind first (the FP64 layout: 256 slots,
+    // 256 faces) runs of the cube-map curve are filled up to the slots, which
+    // fixed-size leaves cannot be.
     L.perm.resize(nf);
     for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
-    std::stable_sort(L.perm.begin(), L.perm.end(), [&](int32_t x, int32_t y) { return code[x] < code[y]; });
-    L.iperm.resize(nf);
-    for (int32_t k = 0; k < nf; ++k) L.iperm[L.perm[k]] = k;
+    bisect(L.perm, 0, size_t(nf), g.fcn, size_t(max_faces));
+    L.iperm = inverse(L.perm);
+    L.blk_face = pack(g, L.perm, L.iperm, slots, max_faces, max_faces);
+    if (int64_t(L.blk_face.size()) - 1 > (int64_t(nf) + max_faces - 1) / max_faces) {
+        std::vector<uint64_t> code(nf);
+        for (int32_t f = 0; f < nf; ++f) code[f] = cube_morton(&g.fcn[3 * size_t(f)]);
+        std::vector<int32_t> perm(nf);
+        for (int32_t f = 0; f < nf; ++f) perm[f] = f;
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return code[x] < code[y]; });
+        std::vector<int32_t> iperm = inverse(perm);
+        std::vector<int32_t> blk = pack(g, perm, iperm, slots, max_faces, 0);
+        if (blk.size() < L.blk_face.size()) {
+            L.perm = std::move(perm);
+            L.iperm = std::move(iperm);
+            L.blk_face = std::move(blk);
+        }
+    }
+    while ((L.blk_face.size() - 1) % 8) L.blk_face.push_back(nf);
 
     L.foff.assign(nf + 1, 0);
     for (int32_t k = 0; k < nf; ++k) L.foff[k + 1] = L.foff[k] + (g.foff[L.perm[k] + 1] - g.foff[L.perm[k]]);
@@ -72,31 +169,6 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
         }
     }
     auto twin_owner = [&](int32_t hn) { return L.iperm[g.he_face[g.twin[L.he_old[hn]]]]; };
-
-    // greedy patches: add faces while own half-edges + halo fit the slots
-    // and the face count stays within max_faces
-    L.blk_face.assign(1, 0);
-    int32_t own = 0, halo = 0;
-    for (int32_t k = 0; k < nf; ++k) {
-        const int32_t val = L.foff[k + 1] - L.foff[k];
-        if (val > slots) throw std::runtime_error("face valence exceeds the block's LDS slots");
-        int32_t dh = 0;
-        const int32_t f0 = L.blk_face.back();
-        for (int32_t hn = L.foff[k]; hn < L.foff[k + 1]; ++hn) {
-            const int32_t o = twin_owner(hn);
-            dh += (o >= f0 && o < k) ? -1 : 1;
-        }
-        if (own + val + halo + dh > slots || k - f0 >= max_faces) {
-            L.blk_face.push_back(k);
-            own = 0;
-            halo = 0;
-            dh = val;
-        }
-        own += val;
-        halo += dh;
-    }
-    L.blk_face.push_back(nf);
-    while ((L.blk_face.size() - 1) % 8) L.blk_face.push_back(nf);
 
     const int32_t nb = L.n_blocks();
     L.twin_slot.assign(2 * size_t(nh), 0);
@@ -181,15 +253,16 @@ void validate_layout(const Geometry& g, const Layout& L) {
             require(he >= 0 && he < nh && (he < h0 || he >= h1), "halo half-edges are outside the block");
         }
         // one-sided fluxes of the mixed kernel: an own half-edge evaluates its
-        // edge when it is the lower-index side or its twin is in the halo, and
-        // also fills its twin's slot when that twin is own; every own slot must
-        // be written exactly once
+        // edge when it is the side of lower mesh index or its twin is in the
+        // halo, and also fills its twin's slot when that twin is own; every
+        // own slot must be written exactly once
         std::vector<int32_t> filled(size_t(n_own), 0);
         for (int32_t hn = h0; hn < h1; ++hn) {
             const int32_t tw = L.twin_slot[2 * size_t(hn)];
             const bool twin_own = tw >= h0 && tw < h1;
-            if (hn < tw || !twin_own) filled[size_t(hn - h0)]++;
-            if (hn < tw && twin_own) filled[size_t(tw - h0)]++;
+            const bool low = L.he_old[hn] < L.he_old[tw];
+            if (low || !twin_own) filled[size_t(hn - h0)]++;
+            if (low && twin_own) filled[size_t(tw - h0)]++;
         }
         for (int32_t q = 0; q < n_own; ++q) require(filled[size_t(q)] == 1, "each own flux written once");
         // face set: own faces first, every local index in range and pointing

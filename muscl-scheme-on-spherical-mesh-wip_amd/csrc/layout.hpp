@@ -1,5 +1,5 @@
-// Internal GPU layout of a mesh: faces reordered along a cube-map Morton curve
-// of their centres, half-edges face-major in that order, and the partition of
+// Internal GPU layout of a mesh: faces reordered by a coordinate bisection or
+// along a cube-map Morton curve of their centres, half-edges face-major in that order, and the partition of
 // faces into patches ("blocks") that one workgroup advances through an RK
 // stage with its half-edges plus the outside twins of its boundary
 // half-edges (the halo) resident in LDS.

@@ -211,7 +211,13 @@ MixedTables pack_mixed_tables(const RunHost& r) {
         eidx.assign(size_t(n_slots), -1);
         own_of_halo.assign(size_t(n_slots), -1);
         auto tw_of = [&](int32_t s) { return L.twin_slot[2 * size_t(h0 + s) + 1]; };
-        auto low_of = [&](int32_t s) { return h0 + s < L.twin_slot[2 * size_t(h0 + s)]; };
+        // the side that evaluates an edge's flux: the half-edge of lower mesh
+        // index, so the choice (and the FP32 rounding) is that of the mesh,
+        // whatever the layout's patch shape and face order
+        auto low_of = [&](int32_t s) {  // own slots s < n_own
+            const int32_t h = L.he_old[h0 + s];
+            return h < r.geo.twin[h];
+        };
         int32_t n_edges = 0;
         for (int32_t s = 0; s < n_own; ++s)  // representatives: the slots that evaluate a flux
             if (low_of(s) || tw_of(s) >= n_own) eidx[size_t(s)] = n_edges++;

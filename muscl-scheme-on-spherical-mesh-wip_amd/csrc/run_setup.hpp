@@ -61,7 +61,7 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces);
 //                      (beta.1 = 1 - beta.0 to 3e-13 on every generated mesh)
 //   slot_b [n_ext][4]: H+/H-, BM_dist/H+, H+/BM_dist, arc/area
 //   edge32 [n_ext/2][8]: unit edge midpoint, |ec|, normal, chord
-//   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower-index << 18
+//   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower mesh index << 18
 //   rate, rd_area [nf]: accretion rate profile, round_diff / area
 struct MixedTables {
     std::vector<uint32_t> slot_a, edge_rec;

@@ -496,9 +496,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     __syncthreads();
 
     // phase 2: one flux per edge, thread k for edge k, evaluated from the
-    // edge's lower-index half-edge (the twin side has the same midpoint and
-    // the negated normal) and written as +F*chord / -F*chord, so the two
-    // faces of an edge exchange exactly opposite fluxes
+    // edge's half-edge of lower mesh index (the twin side has the same
+    // midpoint and the negated normal) and written as +F*chord / -F*chord, so
+    // the two faces of an edge exchange exactly opposite fluxes, whatever the
+    // patch shape
     const int sw = int(rec & 511u);          // the evaluating (own) slot
     const int tws = int((rec >> 9) & 511u);  // its twin's slot
     const bool low = (rec >> 18) & 1u;

@@ -75,7 +75,7 @@ def test_ensemble_rejects_mixed_solver_shapes():
 
 
 def test_patch_shape_and_autotuner_leave_results_unchanged():
-    """Each edge's flux is evaluated from its lower-index half-edge with the
+    """Each edge's flux is evaluated from its half-edge of lower mesh index with the
     same inputs whatever patch holds it, so the state does not depend on the
     patch face cap: fixed caps and the tuned one agree bit for bit."""
     import splash_amd as S
