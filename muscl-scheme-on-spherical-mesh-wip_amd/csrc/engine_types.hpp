@@ -74,14 +74,14 @@ struct GeomPtrs {
     // mixed-precision kernel only (null in FP64 runs).  Per extended slot
     // row (block b's own then halo half-edges start at foff[f0] +
     // halo_off[b]):
-    //   slot_a [n_ext][4]: packed record x, y; beta+0, beta-0 (float bits)
+    //   slot_a [n_ext][2]: packed record x, y
     //     x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30
     //     y = minus1 | owner << 10 | edge << 20 | valence << 28
     //     (face fields: face-set locals)
-    //   slot_b [n_ext][4]: H+/H-, BM_dist/H+, H+/BM_dist, arc/area
+    //   slot_b [n_ext][4]: H+/H-, BM_dist/H+, beta+0, beta-0
     // and per edge of a block (n_slots/2 edges from row (foff[f0] +
     // halo_off[b]) / 2, oriented for the slot that evaluates the flux):
-    //   edge32 [n_ext/2][8]: unit edge midpoint xyz, |ec|, normal xyz, chord
+    //   edge32 [n_ext/2][8]: unit edge midpoint xyz, arc length, normal xyz, chord
     //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | low << 18
     const uint32_t* slot_a;
     const float* slot_b;
@@ -91,6 +91,7 @@ struct GeomPtrs {
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
     const double* rd_area;    // [nf] round_diff / area
+    const float* inv_area;    // [nf] 1 / area (FP32)
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)

@@ -82,7 +82,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     // the trailing padding blocks of the whole grid are empty
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
-    std::vector<float> slot_b, edge32;
+    std::vector<float> slot_b, edge32, inv_area;
     std::vector<double> fcn4, area, rate, rd_area, U;
     std::vector<PhysParams> phys;
     int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1, slot_max = 4;
@@ -94,7 +94,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         const int32_t nf = rh.geo.nf, nh = rh.geo.nh;
         int32_t nb = 0;  // the run's blocks without its own trailing padding
         while (nb < L.n_blocks() && L.blk_face[nb + 1] > L.blk_face[nb]) ++nb;
-        if (int64_t(slot_a.size()) / 4 != int64_t(H) + halo) throw std::logic_error("ensemble row offsets");
+        if (int64_t(slot_a.size()) / 2 != int64_t(H) + halo) throw std::logic_error("ensemble row offsets");
         run_blk.push_back(int32_t(blk_face.size()));
         for (int32_t b = 0; b < nb; ++b) {
             blk_face.push_back(L.blk_face[b] + F);
@@ -110,6 +110,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         append(edge_rec, m.edge_rec);
         append(rate, m.rate);
         append(rd_area, m.rd_area);
+        append(inv_area, m.inv_area);
         append(fcn4, rh.fcn4);
         append(area, rh.area);
         for (int32_t k = 0; k < nf; ++k)
@@ -155,6 +156,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * edge_rec.size(), edge_rec.data()));
         d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
         d.rd_area = static_cast<const double*>(dev(sizeof(double) * rd_area.size(), rd_area.data()));
+        d.inv_area = static_cast<const float*>(dev(sizeof(float) * inv_area.size(), inv_area.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));
         d.fset_max = fset_max;

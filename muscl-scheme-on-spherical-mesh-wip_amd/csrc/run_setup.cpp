@@ -195,7 +195,7 @@ MixedTables pack_mixed_tables(const RunHost& r) {
     const size_t n_ext = L.ext_loc.size() / 4;
     if (n_ext % 2) throw std::runtime_error("odd slot count");
     MixedTables m;
-    m.slot_a.assign(4 * n_ext, 0u);
+    m.slot_a.assign(2 * n_ext, 0u);
     m.slot_b.assign(4 * n_ext, 0.f);
     m.edge32.assign(4 * n_ext, 0.f);  // n_ext/2 edges of 8 floats
     m.edge_rec.assign(n_ext / 2, 0u);
@@ -244,18 +244,15 @@ MixedTables pack_mixed_tables(const RunHost& r) {
             const bool rep = own && (low_of(s) || tw_of(s) >= n_own);
             const uint32_t ei = uint32_t(eidx[size_t(s)]);
             if (ei > 255) throw std::runtime_error("slot record does not fit its fields");
-            m.slot_a[4 * e] = q[0] | q[1] << 10 | q[2] << 20 | (rep ? 0u : 1u) << 30;
-            m.slot_a[4 * e + 1] = q[3] | owner << 10 | ei << 20 | valence << 28;
+            m.slot_a[2 * e] = q[0] | q[1] << 10 | q[2] << 20 | (rep ? 0u : 1u) << 30;
+            m.slot_a[2 * e + 1] = q[3] | owner << 10 | ei << 20 | valence << 28;
             if (rep)
                 m.edge_rec[e0 / 2 + ei] = uint32_t(s) | uint32_t(tw_of(s)) << 9 | (low_of(s) ? 1u : 0u) << 18;
-            const float bp = float(r.beta[4 * hn + 0]), bm = float(r.beta[4 * hn + 2]);
-            std::memcpy(&m.slot_a[4 * e + 2], &bp, 4);
-            std::memcpy(&m.slot_a[4 * e + 3], &bm, 4);
             const double h_p = r.hsc[4 * hn + 0], h_m = r.hsc[4 * hn + 1], bm_dist = r.hsc[4 * hn + 2];
             m.slot_b[4 * e + 0] = float(h_p / h_m);
             m.slot_b[4 * e + 1] = float(bm_dist / h_p);
-            m.slot_b[4 * e + 2] = float(h_p / bm_dist);
-            m.slot_b[4 * e + 3] = float(r.hsc[4 * hn + 3]);
+            m.slot_b[4 * e + 2] = float(r.beta[4 * hn + 0]);
+            m.slot_b[4 * e + 3] = float(r.beta[4 * hn + 2]);
             if (rep) {
                 float* row = &m.edge32[8 * (e0 / 2 + ei)];
                 for (int c = 0; c < 4; ++c) {
@@ -263,16 +260,23 @@ MixedTables pack_mixed_tables(const RunHost& r) {
                     row[4 + c] = float(r.en[4 * hn + c]);
                 }
                 // the FP32 kernel takes |ec| (the recomputed norm of the unit
-                // midpoint) as exactly 1
+                // midpoint) as exactly 1; its place holds the edge's arc
+                // length, the same on both sides (one great-circle distance
+                // of the same two vertices)
                 if (row[3] != 1.0f) throw std::runtime_error("edge midpoint norm is not 1 in FP32 (degenerate edge)");
+                const int32_t h = L.he_old[hn];
+                if (r.geo.arc[h] != r.geo.arc[r.geo.twin[h]]) throw std::runtime_error("edge arc differs between its sides");
+                row[3] = float(r.geo.arc[h]);
             }
         }
     }
     m.rate.resize(nf);
     m.rd_area.resize(nf);
+    m.inv_area.resize(nf);
     for (int32_t k = 0; k < nf; ++k) {
         m.rate[k] = r.acc[8 * size_t(k)];
         m.rd_area[k] = r.geo.round_diff / r.area[k];
+        m.inv_area[k] = float(1.0 / r.area[k]);
     }
     for (size_t q = 0; q + 1 < L.fset_off.size(); ++q) m.fset_max = std::max(m.fset_max, L.fset_off[q + 1] - L.fset_off[q]);
     for (int32_t b = 0; b < nb; ++b) {

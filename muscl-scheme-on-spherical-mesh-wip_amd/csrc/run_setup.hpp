@@ -55,17 +55,19 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces);
 // (a block holds both slots of each of its edges, so n_slots/2 edges from
 // row (foff[f0] + halo_off[b]) / 2), the edge oriented for the slot that
 // evaluates its flux:
-//   slot_a [n_ext][4]: x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30,
+//   slot_a [n_ext][2]: x = plus0 | plus1 << 10 | minus0 << 20 | negate-en << 30,
 //                      y = minus1 | owner << 10 | edge << 20 | valence << 28
-//                      (face-set locals), then beta+0, beta-0 as float bits
-//                      (beta.1 = 1 - beta.0 to 3e-13 on every generated mesh)
-//   slot_b [n_ext][4]: H+/H-, BM_dist/H+, H+/BM_dist, arc/area
-//   edge32 [n_ext/2][8]: unit edge midpoint, |ec|, normal, chord
+//                      (face-set locals)
+//   slot_b [n_ext][4]: H+/H-, BM_dist/H+, beta+0, beta-0 (beta.1 = 1 - beta.0
+//                      to 3e-13 on every generated mesh; H+/BM_dist is the
+//                      reciprocal of the second entry)
+//   edge32 [n_ext/2][8]: unit edge midpoint, arc length, normal, chord
 //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower mesh index << 18
 //   rate, rd_area [nf]: accretion rate profile, round_diff / area
+//   inv_area [nf]: 1 / area (the limiter's Courant number is arc * dt / area)
 struct MixedTables {
     std::vector<uint32_t> slot_a, edge_rec;
-    std::vector<float> slot_b, edge32;
+    std::vector<float> slot_b, edge32, inv_area;
     std::vector<double> rate, rd_area;
     int32_t fset_max = 1;
     int32_t slot_max = 4;  // largest own + halo slot count of a patch, a multiple of 4

@@ -26,10 +26,10 @@ __device__ __forceinline__ float m_pow(float x, float y) {
 // The mixed-precision stage kernel.  It follows k_stage (patches of faces,
 // LDS slots for their half-edges and halo, one face loop) but works on FP32
 // copies: the stage input as one float4 (Sigma, L) row per face plus (dim 5)
-// one pressure per face, staged per block in LDS; per slot a 16-byte packed
-// record (stencil face-set indices, owner, edge, valence, beta) and a
-// float4 of H ratios; per edge a float4 {ec, |ec|}, a float4 {en, chord}
-// and a 4-byte slot record.  The state, the RK combine, the dt limits and
+// one pressure per face, staged per block in LDS with dt / area; per slot
+// an 8-byte packed record (stencil face-set indices, owner, edge, valence)
+// and a float4 {H+/H-, BM/H+, beta+0, beta-0}; per edge a float4 {ec, arc},
+// a float4 {en, chord} and a 4-byte slot record.  The state, the RK combine, the dt limits and
 // reductions and the flux accumulation (Kahan-compensated) stay FP64; the
 // accretion / cooling / friction sources run in FP32 on the FP32 stage
 // input.
@@ -106,18 +106,19 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
 }
 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
-// block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed): li
-// = owner, plus0, plus1, minus0, minus1 local indices; bp, bm = beta+0,
-// beta-0 (the pair's second weight is 1 - the first); hs = {H+/H-, BM/H+,
-// H+/BM, arc/area}; ec = the unit edge midpoint; en = the half-edge's
-// outward normal.
+// block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed; sA:
+// dt / area): li = owner, plus0, plus1, minus0, minus1 local indices; hs =
+// {H+/H-, BM/H+, beta+0, beta-0} (each pair's second weight is 1 - the
+// first); arc = the edge's arc length; ec = the unit edge midpoint; en = the
+// half-edge's outward normal.
 // The slopes stay raw differences d+ = u+ - w, d- = w - u-: the reference's
 // r = (d-/H-) / (d+/H+) is (H+/H-) d-/d+ and its U+ = w + (d+/H+) lim BM is
 // w + d+ (BM/H+) lim.
 template <int DIM>
-__device__ __forceinline__ void reconstruct_lds(float dt, const float4* sW, const float* sP, const float* sC,
-                                                const int* li, int valence, float bp, float bm, float4 hs,
+__device__ __forceinline__ void reconstruct_lds(const float4* sW, const float* sP, const float* sC,
+                                                const float* sA, const int* li, int valence, float4 hs, float arc,
                                                 V3<float> ec, V3<float> en, float* out) {
+    const float bp = hs.z, bm = hs.w;
     float w[5][DIM];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -136,12 +137,14 @@ __device__ __forceinline__ void reconstruct_lds(float dt, const float4* sW, cons
     // v . en with v = (ec x L) / (-Sigma |ec|) (dim 5; |ec| is 1 in FP32) or
     // (ec x L) / -Sigma
     const float cl = dot(cross(ec, v3(wi[1], wi[2], wi[3])), en);
+    const float e = __builtin_amdgcn_rcpf(hs.y);  // H+/BM
+    const float cfl = arc * sA[li[0]];            // dt arc / area
     if constexpr (DIM == 5) {
-        const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
-        limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
+        const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0])) * cfl;
+        limiter32<DIM>(D, hs.x, e, nu, valence, lim);
     } else {
-        const float nu = (1.0f - cl * __builtin_amdgcn_rcpf(wi[0])) * dt * hs.w;
-        limiter32<DIM>(D, hs.x, hs.z, nu, valence, lim);
+        const float nu = (1.0f - cl * __builtin_amdgcn_rcpf(wi[0])) * cfl;
+        limiter32<DIM>(D, hs.x, e, nu, valence, lim);
     }
     // dim 5: slot 4 stays the limited pressure; flux32 forms E_fc from it
 #pragma unroll
@@ -367,16 +370,16 @@ __device__ __forceinline__ void face_terms_mixed(const PhysParams& P, const doub
     if (vc > vmax) vmax = vc;
 }
 
-// One slot's rows: packed record (x, y, beta+0, beta-0), {H+/H-, BM/H+,
-// H+/BM, arc/area} and its edge's {ec, |ec|}, {en, chord}.
+// One slot's rows: packed record (x, y), {H+/H-, BM/H+, beta+0, beta-0} and
+// its edge's {ec, arc}, {en, chord}.
 struct SlotRows {
-    uint4 ra;
+    uint2 ra;
     float4 hs, e4, n4;
 };
 
 __device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, uint32_t edge_base) {
     SlotRows r;
-    r.ra = *row(reinterpret_cast<const uint4*>(g.slot_a), ext);
+    r.ra = *row(reinterpret_cast<const uint2*>(g.slot_a), ext);
     r.hs = *row(reinterpret_cast<const float4*>(g.slot_b), ext);
     const uint32_t edge = edge_base + ((r.ra.y >> 20) & 255u);
     r.e4 = *row(reinterpret_cast<const float4*>(g.edge32), 2 * edge);
@@ -388,14 +391,14 @@ __device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, u
 // minus0 << 20 | negate-en << 30; y = minus1 | owner << 10 | edge << 20 |
 // valence << 28.
 template <int DIM>
-__device__ __forceinline__ void reconstruct_slot(float dt, const float4* sW, const float* sP, const float* sC,
-                                                 const SlotRows& r, float* out) {
-    const uint4 ra = r.ra;
+__device__ __forceinline__ void reconstruct_slot(const float4* sW, const float* sP, const float* sC,
+                                                 const float* sA, const SlotRows& r, float* out) {
+    const uint2 ra = r.ra;
     const int li[5] = {int((ra.y >> 10) & 1023u), int(ra.x & 1023u), int((ra.x >> 10) & 1023u),
                        int((ra.x >> 20) & 1023u), int(ra.y & 1023u)};
     const float sg = (ra.x >> 30) & 1u ? -1.0f : 1.0f;
-    reconstruct_lds<DIM>(dt, sW, sP, sC, li, int(ra.y >> 28), __uint_as_float(ra.z), __uint_as_float(ra.w), r.hs,
-                         v3(r.e4.x, r.e4.y, r.e4.z), v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
+    reconstruct_lds<DIM>(sW, sP, sC, sA, li, int(ra.y >> 28), r.hs, r.e4.w, v3(r.e4.x, r.e4.y, r.e4.z),
+                         v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
 }
 
 template <int DIM, int SOLVER, int STAGE>
@@ -406,13 +409,15 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     // dynamic LDS: the reconstructed half-edge states [slot_max][DIM] (after
     // the flux phase has read them, the same rows hold each own slot's flux
     // times chord length), then the face set's FP32 states (float4 Sigma, L
-    // rows; dim 5: pressures, sound speeds), each sized by the layout
+    // rows; dim 5: pressures, sound speeds) and dt / area, each sized by the
+    // layout
     extern __shared__ float4 lds_dyn[];
     float* const sUP = reinterpret_cast<float*>(lds_dyn);
     float* const sF = sUP;
     float4* const sW = lds_dyn + (g.slot_max * DIM) / 4;     // slot_max is a multiple of 4
     float* const sP = reinterpret_cast<float*>(sW + g.fset_max);  // dim 5: pressure
     float* const sC = sP + g.fset_max;                              // dim 5: sound speed
+    float* const sA = DIM == 5 ? sC + g.fset_max : sP;              // dt / area
     const int b = xcd_block();
     const int run = g.blk_run ? g.blk_run[b] : 0;  // ensembles: this patch's run
     const PhysParams P = Pv[run];
@@ -424,6 +429,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     const int n_slots = n_own + (g.halo_off[b + 1] - hoff);
     const int t = threadIdx.x;
     const double dt = sc->dt;
+    const float dt32 = float(dt);
     const uint32_t ext0 = uint32_t(hbase + hoff), edge_base = ext0 >> 1;
 
     const int i = f0 + t;
@@ -453,6 +459,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
             if (q < n_fs) {
                 const float4 w = *row(W4, f[j]);
                 sW[q] = w;
+                sA[q] = dt32 * *row(g.inv_area, f[j]);
                 if constexpr (DIM == 5) {
                     const float pq = *row(W32, uint32_t(4 * g.nf) + f[j]);
                     sP[q] = pq;
@@ -470,8 +477,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         const int t1 = t + kBlock;
         SlotRows r1;
         if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
-        if (t < n_slots) reconstruct_slot<DIM>(float(dt), sW, sP, sC, r0, &sUP[t * DIM]);
-        if (t1 < n_slots) reconstruct_slot<DIM>(float(dt), sW, sP, sC, r1, &sUP[t1 * DIM]);
+        if (t < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r0, &sUP[t * DIM]);
+        if (t1 < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r1, &sUP[t1 * DIM]);
     }
     // face data of phase 3, requested here so the flux phase hides the
     // loads without holding them through the reconstruction
@@ -611,7 +618,7 @@ namespace {
 
 template <int DIM, int SOLVER>
 void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
-    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 24 : 16) + sizeof(float) * DIM * size_t(a.geom.slot_max);
+    const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 28 : 20) + sizeof(float) * DIM * size_t(a.geom.slot_max);
     if (a.geom.fset_max > 1024 || a.geom.slot_max > kSlots || a.geom.slot_max % 4 || lds > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
