@@ -129,7 +129,7 @@ void Engine::upload(const RunHost& r) {
     d.halo_he = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.halo_he.size(), L.halo_he.data()));
     d.slot_a = d.edge_rec = nullptr;
     d.slot_b = d.edge32 = nullptr;
-    d.acc_rate = d.rd_area = nullptr;
+    d.acc_rate = nullptr;
     d.inv_area = nullptr;
     d.fset_off = d.fset_face = nullptr;
     d.fset_max = 0;
@@ -146,7 +146,6 @@ void Engine::upload(const RunHost& r) {
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * m.edge32.size(), m.edge32.data()));
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * m.edge_rec.size(), m.edge_rec.data()));
         d.acc_rate = static_cast<const double*>(dev(sizeof(double) * m.rate.size(), m.rate.data()));
-        d.rd_area = static_cast<const double*>(dev(sizeof(double) * m.rd_area.size(), m.rd_area.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * m.inv_area.size(), m.inv_area.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_off.size(), L.fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_face.size(), L.fset_face.data()));

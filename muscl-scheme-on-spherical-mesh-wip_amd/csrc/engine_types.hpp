@@ -66,7 +66,7 @@ struct GeomPtrs {
     const double* en;         // [nh][4]: edge normal xyz, chord length
     const int32_t* twin_slot; // [nh][2]: twin half-edge, its LDS slot in the owner's block
     const double* area;       // [nf]
-    const double* fcn;        // [nf][4]: unit face centre xyz, unused
+    const double* fcn;        // [nf][4]: unit face centre xyz, round_diff / area
     const double* acc;        // [nf][8]: accretion rate, omega_acc x r (3), r x (omega_acc x r) (3), unused
     const int32_t* blk_face;  // [n_blocks+1] face range of each stage block
     const int32_t* halo_off;  // [n_blocks+1] range of each block's halo in halo_he
@@ -90,7 +90,6 @@ struct GeomPtrs {
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
-    const double* rd_area;    // [nf] round_diff / area
     const float* inv_area;    // [nf] 1 / area (FP32)
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)

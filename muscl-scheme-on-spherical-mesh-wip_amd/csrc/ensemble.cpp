@@ -83,7 +83,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
     std::vector<float> slot_b, edge32, inv_area;
-    std::vector<double> fcn4, area, rate, rd_area, U;
+    std::vector<double> fcn4, area, rate, U;
     std::vector<PhysParams> phys;
     int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1, slot_max = 4;
     for (int r = 0; r < R; ++r) {
@@ -109,7 +109,6 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         append(edge32, m.edge32);
         append(edge_rec, m.edge_rec);
         append(rate, m.rate);
-        append(rd_area, m.rd_area);
         append(inv_area, m.inv_area);
         append(fcn4, rh.fcn4);
         append(area, rh.area);
@@ -155,7 +154,6 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * edge32.size(), edge32.data()));
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * edge_rec.size(), edge_rec.data()));
         d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
-        d.rd_area = static_cast<const double*>(dev(sizeof(double) * rd_area.size(), rd_area.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * inv_area.size(), inv_area.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));

@@ -131,6 +131,7 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces) {
         const int32_t f = L.perm[k];
         r.area[k] = g.area[f];
         for (int c = 0; c < 3; ++c) r.fcn4[4 * size_t(k) + c] = g.fcn[3 * size_t(f) + c];
+        r.fcn4[4 * size_t(k) + 3] = g.round_diff / g.area[f];
     }
 
     // get_light_curves (adiabatic.hpp:228-333): each observer's flux is
@@ -271,11 +272,9 @@ MixedTables pack_mixed_tables(const RunHost& r) {
         }
     }
     m.rate.resize(nf);
-    m.rd_area.resize(nf);
     m.inv_area.resize(nf);
     for (int32_t k = 0; k < nf; ++k) {
         m.rate[k] = r.acc[8 * size_t(k)];
-        m.rd_area[k] = r.geo.round_diff / r.area[k];
         m.inv_area[k] = float(1.0 / r.area[k]);
     }
     for (size_t q = 0; q + 1 < L.fset_off.size(); ++q) m.fset_max = std::max(m.fset_max, L.fset_off[q + 1] - L.fset_off[q]);

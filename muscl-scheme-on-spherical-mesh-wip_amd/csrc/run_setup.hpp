@@ -35,7 +35,7 @@ struct RunHost {
     std::vector<int32_t> he_fv, sten;             // owner | valence << 27; stencil faces
     std::vector<double> beta, hsc, ec, en;         // [nh][4] each
     // internal face order
-    std::vector<double> area, fcn4, acc;           // [nf], [nf][4], [nf][8]
+    std::vector<double> area, fcn4, acc;           // [nf], [nf][4] (centre, round_diff / area), [nf][8]
     std::vector<double> light_w;                   // [nf][4] (dim 5): observer weights of get_light_curves
     PhysParams phys{};
     RunParams params;
@@ -63,12 +63,12 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces);
 //                      reciprocal of the second entry)
 //   edge32 [n_ext/2][8]: unit edge midpoint, arc length, normal, chord
 //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower mesh index << 18
-//   rate, rd_area [nf]: accretion rate profile, round_diff / area
+//   rate [nf]: accretion rate profile
 //   inv_area [nf]: 1 / area (the limiter's Courant number is arc * dt / area)
 struct MixedTables {
     std::vector<uint32_t> slot_a, edge_rec;
     std::vector<float> slot_b, edge32, inv_area;
-    std::vector<double> rate, rd_area;
+    std::vector<double> rate;
     int32_t fset_max = 1;
     int32_t slot_max = 4;  // largest own + halo slot count of a patch, a multiple of 4
 };

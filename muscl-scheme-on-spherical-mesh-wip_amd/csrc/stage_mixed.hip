@@ -547,7 +547,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
         const double* ui = row(U, uint32_t(i * DIM));
 #pragma unroll
         for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
-        const double rd_area = *row(g.rd_area, uint32_t(i));
+        const double rd_area = *row(g.fcn, uint32_t(4 * i + 3));  // round_diff / area
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
         float src[DIM];
 #pragma unroll
