@@ -136,12 +136,17 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     // layout).  Where the slots bind first (the FP64 layout: 256 slots,
     // 256 faces) runs of the cube-map curve are filled up to the slots, which
     // fixed-size leaves cannot be.
-    L.perm.resize(nf);
-    for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
-    bisect(L.perm, 0, size_t(nf), g.fcn, size_t(max_faces));
-    L.iperm = inverse(L.perm);
-    L.blk_face = pack(g, L.perm, L.iperm, slots, max_faces, max_faces);
-    if (int64_t(L.blk_face.size()) - 1 > (int64_t(nf) + max_faces - 1) / max_faces) {
+    // (Leaves whose own half-edges alone exceed the slots on average cannot
+    // reach the minimum: the bisection is not tried then.)
+    const bool try_bisection = int64_t(max_faces) * nh <= int64_t(slots) * nf;
+    if (try_bisection) {
+        L.perm.resize(nf);
+        for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
+        bisect(L.perm, 0, size_t(nf), g.fcn, size_t(max_faces));
+        L.iperm = inverse(L.perm);
+        L.blk_face = pack(g, L.perm, L.iperm, slots, max_faces, max_faces);
+    }
+    if (!try_bisection || int64_t(L.blk_face.size()) - 1 > (int64_t(nf) + max_faces - 1) / max_faces) {
         std::vector<uint64_t> code(nf);
         for (int32_t f = 0; f < nf; ++f) code[f] = cube_morton(&g.fcn[3 * size_t(f)]);
         std::vector<int32_t> perm(nf);
@@ -149,7 +154,7 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
         std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return code[x] < code[y]; });
         std::vector<int32_t> iperm = inverse(perm);
         std::vector<int32_t> blk = pack(g, perm, iperm, slots, max_faces, 0);
-        if (blk.size() < L.blk_face.size()) {
+        if (!try_bisection || blk.size() < L.blk_face.size()) {
             L.perm = std::move(perm);
             L.iperm = std::move(iperm);
             L.blk_face = std::move(blk);
