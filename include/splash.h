@@ -143,8 +143,10 @@ int splash_set_state(splash_ctx *ctx, const double *U_in, double t, int stop_bit
 int splash_diagnostics(splash_ctx *ctx, double out[5]);
 /* get_light_curves (adiabatic.hpp:228-333), adiabatic solvers only: the
  * pressure-weighted projected flux seen by observers along x, (x+z)/sqrt2,
- * z and -z, out = {flux_0, flux_45, flux_90, flux_180}; one device
- * reduction, no state transfer. */
+ * z and -z, out = {flux_0, flux_45, flux_90, flux_180}.  It and
+ * splash_diagnostics come from one device readout of the state (no state
+ * transfer), made by the first of them after a step and reused by the
+ * others until the next step or splash_set_state. */
 int splash_light_curves(splash_ctx *ctx, double out[4]);
 /* write_t_curl's per-face values (adiabatic.hpp:135-178, isothermal.hpp:
  * 72-110), out[n_faces] in mesh face order: for face i,

@@ -365,8 +365,9 @@ public:
             out << "\n";
         }
     }
-    // adiabatic.hpp:228-333: pressure-weighted projected flux towards four observers
-    // adiabatic.hpp:228-333, evaluated by one device reduction
+    // adiabatic.hpp:228-333: pressure-weighted projected flux towards four
+    // observers, from the state's one device readout (shared with the
+    // per-step diagnostics line)
     std::vector<double> get_light_curves() {
         std::vector<double> out(4);
         check_rc(splash_light_curves(ctx_, out.data()));

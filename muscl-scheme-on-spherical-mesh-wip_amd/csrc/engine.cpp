@@ -63,6 +63,8 @@ void Engine::init(const RunHost& r, const double* U_in) {
     } catch (...) {
         for (void* p : allocations_) (void)hipFree(p);
         allocations_.clear();
+        if (h_readout_) (void)hipHostFree(h_readout_);
+        h_readout_ = nullptr;
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
         throw;
@@ -95,6 +97,7 @@ Engine::~Engine() {
     if (graph_) (void)hipGraphDestroy(graph_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
+    if (h_readout_) (void)hipHostFree(h_readout_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -180,19 +183,18 @@ void Engine::upload(const RunHost& r) {
     s.first_err = ~0ull;
     args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars), &s));
 
-    const int B = fp64::block_size();
-    n_diag_blocks_ = (nf + B - 1) / B;
-    d_partial_ = static_cast<double*>(dev(sizeof(double) * 5 * n_diag_blocks_, nullptr));
-    if (dim_ == 5) {
-        d_light_w_ = static_cast<double*>(dev(sizeof(double) * r.light_w.size(), r.light_w.data()));
-        d_light_part_ = static_cast<double*>(dev(sizeof(double) * 4 * n_diag_blocks_, nullptr));
-    }
+    d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(fp64::readout_blocks(nf)), nullptr));
+    d_readout_ = static_cast<double*>(dev(sizeof(double) * 9, nullptr));
+    if (dim_ == 5) d_light_w_ = static_cast<double*>(dev(sizeof(double) * r.light_w.size(), r.light_w.data()));
+    check(hipHostMalloc(reinterpret_cast<void**>(&h_readout_), sizeof(Readout), hipHostMallocDefault),
+          "hipHostMalloc");
 
     perm_ = L.perm;
     he_old_ = L.he_old;
 }
 
 void Engine::enqueue_prepare() {
+    readout_valid_ = false;
     if (precision_ == kMixed) mixed::enqueue_prepare(args_, stream_);
     else fp64::enqueue_prepare(args_, stream_);
     check(hipGetLastError(), "prepare launch");
@@ -209,6 +211,7 @@ void Engine::enqueue_step(double dt0) {
 }
 
 void Engine::profile(double dt0, int n, double* ms) {
+    readout_valid_ = false;
     // n steps launched back to back on this run's stream, each kernel with
     // start/stop HIP events bound to its own dispatch, one wait at the end
     constexpr int K = kKernelsPerStep;
@@ -243,6 +246,7 @@ void Engine::profile(double dt0, int n, double* ms) {
 
 void Engine::step(double dt0) {
     stepped_ = true;
+    readout_valid_ = false;
     enqueue_step(dt0);
     check(hipGetLastError(), "step launch");
 }
@@ -250,6 +254,7 @@ void Engine::step(double dt0) {
 void Engine::steps(double dt0, int n) {
     if (n <= 0) return;
     stepped_ = true;
+    readout_valid_ = false;
     constexpr int kGraphSteps = 8;
     if (n >= kGraphSteps) {
         if (!graph_exec_ || graph_dt0_ != dt0 || graph_steps_ != kGraphSteps) {
@@ -276,10 +281,23 @@ void Engine::steps(double dt0, int n) {
 void Engine::synchronize() { check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
 
 Scalars Engine::read_scalars() {
-    Scalars s;
-    check(hipMemcpyAsync(&s, args_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_), "read scalars");
+    if (readout_valid_) return h_readout_->sc;
+    check(hipMemcpyAsync(&h_readout_->sc, args_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_),
+          "read scalars");
     synchronize();
-    return s;
+    return h_readout_->sc;
+}
+
+void Engine::ensure_readout() {
+    if (readout_valid_) return;
+    fp64::enqueue_readout(args_, d_light_w_, d_readout_part_, d_readout_, stream_);
+    check(hipGetLastError(), "readout launch");
+    check(hipMemcpyAsync(h_readout_->sums, d_readout_, sizeof(h_readout_->sums), hipMemcpyDeviceToHost, stream_),
+          "read readout");
+    check(hipMemcpyAsync(&h_readout_->sc, args_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_),
+          "read scalars");
+    synchronize();
+    readout_valid_ = true;
 }
 
 double Engine::time() { return read_scalars().t; }
@@ -306,6 +324,7 @@ void Engine::get_state(double* U_out) {
 void Engine::set_state(const double* U_in, double t, int stop_bits) {
     if (!U_in) throw std::invalid_argument("U_in is null");
     synchronize();  // steps in flight still read the old state
+    readout_valid_ = false;
     std::vector<double> U(size_t(nf_) * dim_);
     for (int32_t k = 0; k < nf_; ++k)
         for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm_[k]) * dim_ + c];
@@ -321,29 +340,14 @@ void Engine::set_state(const double* U_in, double t, int stop_bits) {
 }
 
 void Engine::diagnostics(double out[5]) {
-    fp64::enqueue_diag(args_, d_partial_, stream_);
-    check(hipGetLastError(), "diag launch");
-    std::vector<double> part(5 * size_t(n_diag_blocks_));
-    check(hipMemcpyAsync(part.data(), d_partial_, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream_),
-          "read diagnostics");
-    synchronize();
-    for (int k = 0; k < 5; ++k) out[k] = 0;
-    for (int b = 0; b < n_diag_blocks_; ++b)
-        for (int k = 0; k < 5; ++k) out[k] += part[5 * size_t(b) + k];
+    ensure_readout();
+    for (int k = 0; k < 5; ++k) out[k] = h_readout_->sums[k];
 }
 
 void Engine::light_curves(double out[4]) {
     if (dim_ != 5) throw std::invalid_argument("light curves exist for the adiabatic (dim 5) solvers only");
-    fp64::enqueue_light(args_, d_light_w_, d_light_part_, stream_);
-    check(hipGetLastError(), "light curve launch");
-    std::vector<double> part(4 * size_t(n_diag_blocks_));
-    check(hipMemcpyAsync(part.data(), d_light_part_, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
-                         stream_),
-          "read light curves");
-    synchronize();
-    for (int k = 0; k < 4; ++k) out[k] = 0;
-    for (int b = 0; b < n_diag_blocks_; ++b)
-        for (int k = 0; k < 4; ++k) out[k] += part[4 * size_t(b) + k];
+    ensure_readout();
+    for (int k = 0; k < 4; ++k) out[k] = h_readout_->sums[5 + k];
 }
 
 void Engine::curl(double* out) {

@@ -78,6 +78,9 @@ private:
     void init(const RunHost& r, const double* U_in);
     void upload(const RunHost& r);
     Scalars read_scalars();
+    // the diagnostics sums, light curves and scalars of the current state,
+    // read once per state: the driver's per-step queries share one readout
+    void ensure_readout();
 
     Geometry geo_;
     int dim_, solver_, precision_;
@@ -94,10 +97,17 @@ private:
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;
     StepArgs args_{};
-    double* d_partial_ = nullptr;
     double* d_light_w_ = nullptr;       // [nf][4] observer weights (dim 5)
-    double* d_light_part_ = nullptr;    // [diag blocks][4]
-    int n_diag_blocks_ = 0;
+    double* d_readout_part_ = nullptr;  // [readout blocks][9] per-block sums
+    double* d_readout_ = nullptr;       // [9] diagnostics sums, then light curves
+    // pinned host copy of the last readout and of the scalars; readout_valid_
+    // holds until the next launch or upload that changes the state
+    struct Readout {
+        double sums[9];
+        Scalars sc;
+    };
+    Readout* h_readout_ = nullptr;
+    bool readout_valid_ = false;
     bool stepped_ = false;              // a step ran since the state was set
     double* d_curl_ = nullptr;          // lazily: ev [nh][3], out [nf], W_tmp [nf][dim]
 

@@ -31,9 +31,12 @@ namespace fp64 {
 void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev = nullptr);
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
-void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s);
-// per-block partial sums [ceil(nf/block_size())][4] of the light curves (dim 5)
-void enqueue_light(const StepArgs& a, const double* weights, double* partial, hipStream_t s);
+// The diagnostics sums and (dim 5) the light curves of the current state
+// into out[9] = {Sigma A, L A (3), E A, flux_0, flux_45, flux_90, flux_180}:
+// two launches, with per-block sums in partial[readout_blocks(nf)][9].
+// light_w: [nf][4] observer weights (dim 5; unused for dim 4).
+int readout_blocks(int nf);
+void enqueue_readout(const StepArgs& a, const double* light_w, double* partial, double* out, hipStream_t s);
 // write_t_curl per face [nf] from the last stage-2 input (W2, or W32b widened
 // into W_tmp [nf][dim]); stepped = false gives the value before any step.
 // ev = v_j - v_j1 per half-edge [nh][3].

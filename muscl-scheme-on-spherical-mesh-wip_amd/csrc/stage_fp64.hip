@@ -49,15 +49,13 @@ void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const Ker
 }
 
 
-void enqueue_diag(const StepArgs& a, double* partial, hipStream_t s) {
-    const int gf = (a.geom.nf + kBlock - 1) / kBlock;
-    if (a.phys.dim == 5) k_diag_partial<5><<<gf, kBlock, 0, s>>>(a.geom, a.U, partial);
-    else k_diag_partial<4><<<gf, kBlock, 0, s>>>(a.geom, a.U, partial);
-}
+int readout_blocks(int nf) { return (nf + kBlock - 1) / kBlock; }
 
-void enqueue_light(const StepArgs& a, const double* weights, double* partial, hipStream_t s) {
-    const int gf = (a.geom.nf + kBlock - 1) / kBlock;
-    k_light_partial<<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, weights, partial);
+void enqueue_readout(const StepArgs& a, const double* light_w, double* partial, double* out, hipStream_t s) {
+    const int gf = readout_blocks(a.geom.nf);
+    if (a.phys.dim == 5) k_readout_partial<5><<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, light_w, partial);
+    else k_readout_partial<4><<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, light_w, partial);
+    k_readout_final<<<1, kBlock, 0, s>>>(partial, gf, out);
 }
 
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s) {

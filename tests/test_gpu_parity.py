@@ -293,6 +293,33 @@ def test_light_curves_device_reduction(precision):
 
 
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_driver_readout_is_one_per_state(precision):
+    """The diagnostics sums, light curves and scalars come from one device
+    readout per state: repeated queries agree bit for bit, a step replaces
+    them (the light curves then match the stepped state), and after set_state
+    they equal those of a fresh solver built on the same state."""
+    import splash_amd as S
+    mesh = S.icosphere_hex(4)
+    prec = getattr(S, precision)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=8, omega=(0, 0, 0.2))
+    U1 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=9, omega=(0.1, 0, 0.3))
+    a = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
+    d0, l0 = a.diagnostics(), a.get_light_curves()
+    assert np.array_equal(a.diagnostics(), d0) and np.array_equal(a.get_light_curves(), l0)
+    a.do_step(1e-4)
+    assert a.time() > 0
+    g = S.Geometry(mesh)
+    l1 = a.get_light_curves()
+    assert not np.array_equal(l1, l0)
+    np.testing.assert_allclose(l1, light_curves(a.get_state(), g.fc, g.area, GAM), rtol=1e-12)
+    a.set_state(U1, 0.5, 0)
+    b = S.MUSCL_HLLCplus(mesh, U1, 5, GAM, 0.0, precision=prec)
+    assert np.array_equal(a.diagnostics(), b.diagnostics())
+    assert np.array_equal(a.get_light_curves(), b.get_light_curves())
+    assert a.time() == 0.5 and b.time() == 0.0
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
 def test_set_state_resumes_exactly(precision):
     """Resuming from (state, time, stop bits) continues bit for bit as the
     run that produced them."""
