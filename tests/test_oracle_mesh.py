@@ -1,9 +1,9 @@
 """Pins of the oracle's mesh generators and geometry precompute.
 
-Known-answer values measured on the reference build during the survey of
-the reference (ΣA-4π on quad_sphere(4), h0 on icosphere_hex(5), area_coeff
-on quad_sphere(4), face counts) plus invariants and closed forms that an
-indexing, sign or operand mistake would break.
+Known-answer values measured on a build of the real reference (ΣA-4π on
+quad_sphere(4), h0 on icosphere_hex(5), area_coeff on quad_sphere(4), face
+counts) plus invariants and closed forms that an indexing, sign or operand
+mistake would break.
 """
 import math
 
@@ -83,8 +83,8 @@ def test_icosahedron_level0_closed_form():
         assert got[k:] + got[:k] == want
 
 
-def test_survey_known_answers():
-    """Values the reference build produced in the survey's probes."""
+def test_reference_known_answers():
+    """Values a build of the real reference produced."""
     q4 = O.Mesh("quad", 4)
     g = O.Geometry(q4)
     assert abs(g.area.sum() - 4 * np.pi) == pytest.approx(1.7e-12, rel=0.05)

@@ -6,8 +6,8 @@ Dynamics", 3rd ed.: PVRS wave speeds eqs. 10.67-10.69 with Davis/Einfeldt
 style bounds, HLL flux eq. 10.21, standard HLLC eqs. 10.37-10.41); the
 angular-momentum slots are r x (momentum flux).  The step is checked by
 invariants: conservation with sources off, exact rest-state preservation,
-thread-count independence, the CFL clamp and the stop_check triggers the
-survey measured on the reference build.
+thread-count independence, the CFL clamp and the stop_check triggers
+measured on a build of the real reference.
 """
 import math
 
@@ -259,8 +259,8 @@ def test_cfl_clamp_uses_max_of_speed_and_sound_speed():
 
 def test_stop_check_triggers():
     # solid-body rotation with Omega = 0.5 makes g_eff = GM - |v|^2 < 0 at the
-    # equator: the friction dt limit goes negative on step 1 (the survey saw
-    # this on the reference build for icosphere_hex(5)); Omega = 0.3 does not.
+    # equator: the friction dt limit goes negative on step 1 (a build of the
+    # real reference does so for icosphere_hex(5)); Omega = 0.3 does not.
     m = O.Mesh("hex", 5)
     for om, expect in ((0.5, True), (0.3, False)):
         U, _ = _rotating(m, 5, om)
