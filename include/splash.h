@@ -221,6 +221,13 @@ int splash_ensemble_time(splash_ensemble *e, int run, double *t);
 int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
 int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);
 int splash_ensemble_get_state(splash_ensemble *e, int run, double *U_out);
+/* Run `run`'s splash_diagnostics sums (MUSCL_base.hpp:187-205) and, for
+ * adiabatic runs, splash_light_curves (adiabatic.hpp:228-333).  One device
+ * readout covers every run and serves until the next splash_ensemble_step;
+ * each run's values equal those of a single SPLASH_MIXED solver in the same
+ * state with the same patch shape, bit for bit. */
+int splash_ensemble_diagnostics(splash_ensemble *e, int run, double out[5]);
+int splash_ensemble_light_curves(splash_ensemble *e, int run, double out[4]);
 
 /* Host-side check of the internal GPU layout of a mesh that the
  * mixed-precision kernel uses (coordinate-bisection face order, patches of at most 64

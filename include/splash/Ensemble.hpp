@@ -70,6 +70,17 @@ public:
         check_rc(splash_ensemble_stop_check(e_, run, &s));
         return s != 0;
     }
+    // run's do_step diagnostics sums and light curves (one readout of every run)
+    std::vector<double> diagnostics(int run) {
+        std::vector<double> out(5);
+        check_rc(splash_ensemble_diagnostics(e_, run, out.data()));
+        return out;
+    }
+    std::vector<double> get_light_curves(int run) {
+        std::vector<double> out(4);
+        check_rc(splash_ensemble_light_curves(e_, run, out.data()));
+        return out;
+    }
     std::vector<std::vector<double>> get_state(int run) {
         const size_t nf = faces_.at(size_t(run));
         std::vector<double> flat(nf * size_t(dim_));

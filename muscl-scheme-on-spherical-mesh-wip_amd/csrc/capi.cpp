@@ -481,6 +481,26 @@ int splash_ensemble_get_state(splash_ensemble* e, int run, double* U_out) {
     });
 }
 
+int splash_ensemble_diagnostics(splash_ensemble* e, int run, double* out) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        e->ens->diagnostics(run, out);
+        return SPLASH_OK;
+    });
+}
+
+int splash_ensemble_light_curves(splash_ensemble* e, int run, double* out) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        e->ens->light_curves(run, out);
+        return SPLASH_OK;
+    });
+}
+
 const char* splash_last_error_message(void) { return g_last_error.c_str(); }
 
 }  // extern "C"

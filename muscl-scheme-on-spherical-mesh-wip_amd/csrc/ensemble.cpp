@@ -83,7 +83,10 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
     std::vector<float> slot_b, edge32, inv_area;
-    std::vector<double> fcn4, area, rate, U;
+    std::vector<double> fcn4, area, rate, U, light_w;
+    std::vector<int2> seg;  // readout workgroups: each run's faces in blocks from its first face
+    std::vector<int32_t> seg_run, run_part;
+    const int32_t kReadoutFaces = fp64::block_size();
     std::vector<PhysParams> phys;
     int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1, slot_max = 4;
     for (int r = 0; r < R; ++r) {
@@ -112,6 +115,12 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         append(inv_area, m.inv_area);
         append(fcn4, rh.fcn4);
         append(area, rh.area);
+        if (dim == 5) append(light_w, rh.light_w);
+        run_part.push_back(int32_t(seg.size()));
+        for (int32_t f = 0; f < nf; f += kReadoutFaces) {
+            seg.push_back(make_int2(F + f, F + std::min(nf, f + kReadoutFaces)));
+            seg_run.push_back(r);
+        }
         for (int32_t k = 0; k < nf; ++k)
             for (int c = 0; c < dim; ++c) U.push_back(U_in[size_t(r)][size_t(L.perm[k]) * dim + c]);
         phys.push_back(rh.phys);
@@ -125,6 +134,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         halo += int32_t(L.halo_he.size());
         fsets += int32_t(L.fset_face.size());
     }
+    run_part.push_back(int32_t(seg.size()));
     run_blk.push_back(int32_t(blk_face.size()));
     blk_face.push_back(F);
     halo_off.push_back(halo);
@@ -182,6 +192,15 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars) * sc.size(), sc.data()));
         args_.U = static_cast<double*>(dev(sizeof(double) * U.size(), U.data()));
         args_.W = args_.W2 = nullptr;
+        n_seg_ = int(seg.size());
+        d_seg_ = static_cast<const int2*>(dev(sizeof(int2) * seg.size(), seg.data()));
+        d_seg_run_ = static_cast<const int*>(dev(sizeof(int32_t) * seg_run.size(), seg_run.data()));
+        d_run_part_ = static_cast<const int*>(dev(sizeof(int32_t) * run_part.size(), run_part.data()));
+        if (dim == 5) d_light_w_ = static_cast<double*>(dev(sizeof(double) * light_w.size(), light_w.data()));
+        d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * seg.size(), nullptr));
+        d_readout_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(R), nullptr));
+        readout_.assign(9 * size_t(R), 0.0);
+        scalars_.assign(size_t(R), Scalars{});
         const size_t row = dim == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
@@ -226,6 +245,7 @@ void Ensemble::enqueue_step() {
 void Ensemble::steps(const double* dt0, int n) {
     upload_dt0(dt0);
     if (n <= 0) return;
+    readout_valid_ = false;
     constexpr int kGraphSteps = 8;
     if (n >= kGraphSteps) {
         if (!graph_exec_) {  // dt0 lives in device memory, so one graph serves every call
@@ -245,6 +265,7 @@ void Ensemble::steps(const double* dt0, int n) {
 
 void Ensemble::profile(const double* dt0, int n, double* ms) {
     upload_dt0(dt0);
+    readout_valid_ = false;
     constexpr int K = kKernelsPerStep;
     const size_t nk = size_t(n > 0 ? n : 0) * K;
     std::vector<hipEvent_t> ev(2 * nk, nullptr);
@@ -279,10 +300,39 @@ void Ensemble::synchronize() { check(hipStreamSynchronize(stream_), "hipStreamSy
 
 Scalars Ensemble::read_scalars(int r) {
     if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    if (readout_valid_) return scalars_[size_t(r)];
     Scalars s;
     check(hipMemcpyAsync(&s, args_.sc + r, sizeof(Scalars), hipMemcpyDeviceToHost, stream_), "read scalars");
     synchronize();
     return s;
+}
+
+void Ensemble::ensure_readout() {
+    if (readout_valid_) return;
+    fp64::enqueue_readout_runs(args_, d_light_w_, d_seg_, d_seg_run_, d_run_part_, n_seg_, d_readout_part_, d_readout_,
+                               stream_);
+    check(hipGetLastError(), "readout launch");
+    check(hipMemcpyAsync(readout_.data(), d_readout_, sizeof(double) * readout_.size(), hipMemcpyDeviceToHost,
+                         stream_),
+          "read readout");
+    check(hipMemcpyAsync(scalars_.data(), args_.sc, sizeof(Scalars) * scalars_.size(), hipMemcpyDeviceToHost,
+                         stream_),
+          "read scalars");
+    synchronize();
+    readout_valid_ = true;
+}
+
+void Ensemble::diagnostics(int r, double out[5]) {
+    if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    ensure_readout();
+    for (int k = 0; k < 5; ++k) out[k] = readout_[9 * size_t(r) + k];
+}
+
+void Ensemble::light_curves(int r, double out[4]) {
+    if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    if (dim_ != 5) throw std::invalid_argument("light curves exist for the adiabatic (dim 5) solvers only");
+    ensure_readout();
+    for (int k = 0; k < 4; ++k) out[k] = readout_[9 * size_t(r) + 5 + k];
 }
 
 double Ensemble::time(int r) { return read_scalars(r).t; }

@@ -48,6 +48,11 @@ public:
     double last_dt(int r);
     int stop_bits(int r);
     void get_state(int r, double* U_out);
+    // run r's do_step diagnostics sums and (dim 5) light curves, as
+    // Engine::diagnostics / light_curves: one device readout covers every
+    // run and serves until the next step
+    void diagnostics(int r, double out[5]);
+    void light_curves(int r, double out[4]);
 
 private:
     Ensemble(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in, int dim, int solver);
@@ -56,6 +61,7 @@ private:
     void enqueue_step();
     void enqueue_kernel(int which, const KernelEvents* ev = nullptr);
     Scalars read_scalars(int r);
+    void ensure_readout();
     void* dev(size_t bytes, const void* host);
 
     int dim_, solver_;
@@ -68,6 +74,19 @@ private:
     std::vector<void*> allocations_;
     StepArgs args_{};
     double* dt0_dev_ = nullptr;
+    // readout: per-workgroup face ranges and runs, each run's first
+    // workgroup, per-workgroup sums, per-run results; the host copies
+    // (sums [n_runs][9], then the scalars) stay valid until the next launch
+    const int2* d_seg_ = nullptr;
+    const int* d_seg_run_ = nullptr;
+    const int* d_run_part_ = nullptr;
+    int n_seg_ = 0;
+    double* d_light_w_ = nullptr;
+    double* d_readout_part_ = nullptr;
+    double* d_readout_ = nullptr;
+    std::vector<double> readout_;
+    std::vector<Scalars> scalars_;
+    bool readout_valid_ = false;
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

@@ -52,10 +52,15 @@ void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const Ker
 int readout_blocks(int nf) { return (nf + kBlock - 1) / kBlock; }
 
 void enqueue_readout(const StepArgs& a, const double* light_w, double* partial, double* out, hipStream_t s) {
-    const int gf = readout_blocks(a.geom.nf);
-    if (a.phys.dim == 5) k_readout_partial<5><<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, light_w, partial);
-    else k_readout_partial<4><<<gf, kBlock, 0, s>>>(a.geom, a.phys.gam, a.U, light_w, partial);
-    k_readout_final<<<1, kBlock, 0, s>>>(partial, gf, out);
+    enqueue_readout_runs(a, light_w, nullptr, nullptr, nullptr, readout_blocks(a.geom.nf), partial, out, s);
+}
+
+void enqueue_readout_runs(const StepArgs& a, const double* light_w, const int2* seg, const int* seg_run,
+                          const int* run_part, int n_part, double* partial, double* out, hipStream_t s) {
+    if (a.phys.dim == 5)
+        k_readout_partial<5><<<n_part, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, light_w, seg, seg_run, partial);
+    else k_readout_partial<4><<<n_part, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, light_w, seg, seg_run, partial);
+    k_readout_final<<<run_part ? a.geom.n_runs : 1, kBlock, 0, s>>>(partial, run_part, n_part, out);
 }
 
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s) {

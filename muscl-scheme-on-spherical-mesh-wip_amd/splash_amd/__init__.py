@@ -343,6 +343,17 @@ class Ensemble:
         check(lib().splash_ensemble_stop_check(self._e, int(run), C.byref(s)))
         return bool(s.value)
 
+    def diagnostics(self, run):
+        """Run `run`'s diagnostics sums (one device readout of every run)."""
+        out = np.empty(5)
+        check(lib().splash_ensemble_diagnostics(self._e, int(run), _dp(out)))
+        return out
+
+    def get_light_curves(self, run):
+        out = np.empty(4)
+        check(lib().splash_ensemble_light_curves(self._e, int(run), _dp(out)))
+        return out
+
     def get_state(self, run):
         n = C.c_int32()
         check(lib().splash_ensemble_num_faces(self._e, int(run), C.byref(n)))

@@ -50,6 +50,29 @@ def test_ensemble_runs_equal_single_mixed_runs():
         assert ens.get_stop_check(r) == s.get_stop_check()
 
 
+def test_ensemble_readout_equals_single_runs():
+    """Each run's diagnostics sums and light curves from the ensemble's one
+    readout (a run's faces in blocks from its own first face, its own gam)
+    equal a single solver's bit for bit, before stepping and after; the
+    readout follows the steps."""
+    import splash_amd as S
+    runs = _runs(S)
+    ens = S.Ensemble([(m, U0, g, om, p) for m, U0, g, om, p, _ in runs], 5, S.HLLCPLUS)
+    singles = [S.MUSCL_HLLCplus(m, U0, 5, g, om, p is not None, params=p, precision=S.MIXED)
+               for m, U0, g, om, p, _ in runs]
+    dt0 = [r[5] for r in runs]
+    before = [ens.get_light_curves(r) for r in range(len(runs))]
+    for step in (0, 11):
+        ens.do_steps(dt0, step)
+        for s, d in zip(singles, dt0):
+            s.do_steps(d, step)
+        for r, s in enumerate(singles):
+            assert np.array_equal(ens.diagnostics(r), s.diagnostics())
+            assert np.array_equal(ens.get_light_curves(r), s.get_light_curves())
+            assert ens.time(r) == s.time()
+    assert not any(np.array_equal(ens.get_light_curves(r), before[r]) for r in range(len(runs)))
+
+
 def test_ensemble_isothermal_and_changing_dt0():
     """dim 4 / MUSCL_HLLE runs, and a per-run dt0 that changes between calls."""
     import splash_amd as S
