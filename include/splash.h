@@ -221,6 +221,10 @@ int splash_ensemble_time(splash_ensemble *e, int run, double *t);
 int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
 int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);
 int splash_ensemble_get_state(splash_ensemble *e, int run, double *U_out);
+/* Resume of one run, as splash_set_state (no counterpart in the reference):
+ * replaces run `run`'s state, time and stop_check bits; the other runs
+ * continue exactly as they would have. */
+int splash_ensemble_set_state(splash_ensemble *e, int run, const double *U_in, double t, int stop_bits);
 /* Run `run`'s splash_diagnostics sums (MUSCL_base.hpp:187-205) and, for
  * adiabatic runs, splash_light_curves (adiabatic.hpp:228-333).  One device
  * readout covers every run and serves until the next splash_ensemble_step;

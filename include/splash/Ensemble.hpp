@@ -70,6 +70,17 @@ public:
         check_rc(splash_ensemble_stop_check(e_, run, &s));
         return s != 0;
     }
+    // resume one run from a state in mesh face order, its time and stop bits
+    void set_state(int run, const std::vector<std::vector<double>>& U, double t, int stop_bits = 0) {
+        const size_t nf = faces_.at(size_t(run));
+        if (U.size() != nf) throw std::invalid_argument("one state row per face");
+        std::vector<double> flat(nf * size_t(dim_));
+        for (size_t i = 0; i < nf; ++i) {
+            if (U[i].size() != size_t(dim_)) throw std::invalid_argument("state rows of dim values");
+            for (int k = 0; k < dim_; ++k) flat[i * size_t(dim_) + size_t(k)] = U[i][size_t(k)];
+        }
+        check_rc(splash_ensemble_set_state(e_, run, flat.data(), t, stop_bits));
+    }
     // run's do_step diagnostics sums and light curves (one readout of every run)
     std::vector<double> diagnostics(int run) {
         std::vector<double> out(5);

@@ -481,6 +481,16 @@ int splash_ensemble_get_state(splash_ensemble* e, int run, double* U_out) {
     });
 }
 
+int splash_ensemble_set_state(splash_ensemble* e, int run, const double* U_in, double t, int stop_bits) {
+    ENS_OR_FAIL;
+    RUN_OR_FAIL;
+    if (!U_in) return fail(SPLASH_EINVAL, "null state");
+    return guarded([&] {
+        e->ens->set_state(run, U_in, t, stop_bits);
+        return SPLASH_OK;
+    });
+}
+
 int splash_ensemble_diagnostics(splash_ensemble* e, int run, double* out) {
     ENS_OR_FAIL;
     RUN_OR_FAIL;

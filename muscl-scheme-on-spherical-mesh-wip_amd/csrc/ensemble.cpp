@@ -339,6 +339,31 @@ double Ensemble::time(int r) { return read_scalars(r).t; }
 double Ensemble::last_dt(int r) { return read_scalars(r).dt; }
 int Ensemble::stop_bits(int r) { return read_scalars(r).stop; }
 
+void Ensemble::set_state(int r, const double* U_in, double t, int stop_bits) {
+    if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
+    if (!U_in) throw std::invalid_argument("U_in is null");
+    synchronize();  // steps in flight still read the old state
+    readout_valid_ = false;
+    const int32_t nf = faces_[size_t(r)];
+    const std::vector<int32_t>& perm = perm_[size_t(r)];
+    std::vector<double> U(size_t(nf) * dim_);
+    for (int32_t k = 0; k < nf; ++k)
+        for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm[k]) * dim_ + c];
+    check(hipMemcpy(args_.U + size_t(face_off_[size_t(r)]) * dim_, U.data(), U.size() * sizeof(double),
+                    hipMemcpyHostToDevice),
+          "upload state");
+    Scalars s = read_scalars(r);
+    s.t = t;
+    s.stop = stop_bits;
+    s.first_err = ~0ull;
+    check(hipMemcpy(args_.sc + r, &s, sizeof(Scalars), hipMemcpyHostToDevice), "upload scalars");
+    // stage inputs and dt partials of every run: the prepare kernel uses the
+    // stage epilogue's arithmetic, so the other runs' values do not change
+    mixed::enqueue_prepare(args_, stream_);
+    check(hipGetLastError(), "prepare launch");
+    synchronize();
+}
+
 void Ensemble::get_state(int r, double* U_out) {
     if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
     const int32_t nf = faces_[size_t(r)];

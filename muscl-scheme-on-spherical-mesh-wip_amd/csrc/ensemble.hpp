@@ -48,6 +48,9 @@ public:
     double last_dt(int r);
     int stop_bits(int r);
     void get_state(int r, double* U_out);
+    // resume run r as Engine::set_state does: its state (mesh face order),
+    // time and sticky stop bits; the other runs continue unchanged
+    void set_state(int r, const double* U_in, double t, int stop_bits);
     // run r's do_step diagnostics sums and (dim 5) light curves, as
     // Engine::diagnostics / light_curves: one device readout covers every
     // run and serves until the next step

@@ -343,6 +343,15 @@ class Ensemble:
         check(lib().splash_ensemble_stop_check(self._e, int(run), C.byref(s)))
         return bool(s.value)
 
+    def set_state(self, run, U, t, stop_bits=0):
+        """Resume run `run`: state (mesh face order), time, stop bits."""
+        n = C.c_int32()
+        check(lib().splash_ensemble_num_faces(self._e, int(run), C.byref(n)))
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (n.value, self._dim):
+            raise ValueError(f"state must be [{n.value}][{self._dim}]")
+        check(lib().splash_ensemble_set_state(self._e, int(run), _dp(U), float(t), int(stop_bits)))
+
     def diagnostics(self, run):
         """Run `run`'s diagnostics sums (one device readout of every run)."""
         out = np.empty(5)

@@ -94,6 +94,7 @@ SIGNATURES = {
     "splash_ensemble_last_dt": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_ensemble_stop_check": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "splash_ensemble_get_state": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "splash_ensemble_set_state": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_double, C.c_int]),
     "splash_ensemble_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_ensemble_light_curves": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_last_error_message": (C.c_char_p, []),

@@ -73,6 +73,28 @@ def test_ensemble_readout_equals_single_runs():
     assert not any(np.array_equal(ens.get_light_curves(r), before[r]) for r in range(len(runs)))
 
 
+def test_ensemble_set_state_resumes_exactly():
+    """A fresh ensemble whose runs are resumed one by one from another's
+    states, times and stop_check after 13 steps continues bit for bit as the
+    uninterrupted one over 11 more steps (graph replay plus eager steps)."""
+    import splash_amd as S
+    runs = _runs(S)
+    spec = [(m, U0, g, om, p) for m, U0, g, om, p, _ in runs]
+    dt0 = [r[5] for r in runs]
+    a = S.Ensemble(spec, 5, S.HLLCPLUS)
+    a.do_steps(dt0, 13)
+    saved = [(a.get_state(r), a.time(r), int(a.get_stop_check(r))) for r in range(len(runs))]
+    b = S.Ensemble(spec, 5, S.HLLCPLUS)
+    for r, (U, t, stop) in enumerate(saved):
+        b.set_state(r, U, t, stop)
+    a.do_steps(dt0, 11)
+    b.do_steps(dt0, 11)
+    for r in range(len(runs)):
+        assert np.array_equal(a.get_state(r), b.get_state(r))
+        assert a.time(r) == b.time(r) and a.last_dt(r) == b.last_dt(r)
+        assert a.get_stop_check(r) == b.get_stop_check(r)
+
+
 def test_ensemble_isothermal_and_changing_dt0():
     """dim 4 / MUSCL_HLLE runs, and a per-run dt0 that changes between calls."""
     import splash_amd as S
