@@ -14,7 +14,11 @@ configurations (c2: icosphere_hex(7) HLLC+; c5a / c5b: uv_sphere(2048,1024)
 Both precision paths run the same steps from the same state:
   value       = face-updates/s of the FP32-flux / FP64-state path,
   fp64        = face-updates/s of the plain FP64 path,
-  l1_err      = relative L1 density error of the FP32-flux path vs FP64.
+  l1_err      = relative L1 density error of the FP32-flux path vs FP64,
+  driver_loop = ms per iteration of the reference driver's per-step loop
+                (do_step, the diagnostics line's sums, time(),
+                get_light_curves()) on the FP32-flux path, after the timed
+                region.
 The roofline object covers the dominant kernel of the FP32-flux path, timed
 with HIP events on the run's stream, against the 8 TB/s peak and against a
 device-to-device copy timed on the same GPU (copy_GBs); the CPU baseline is
@@ -105,6 +109,31 @@ def profile(s, dt0, n):
     an idle gap it needs several milliseconds to return to it)."""
     s.do_steps(dt0, 64)
     return s.profile_step(dt0, n)
+
+
+def driver_loop(s, dt0, n):
+    """The reference driver's per-step pattern (test.cpp's loop): do_step,
+    then the diagnostics line's sums, time() and get_light_curves(), one
+    eager step at a time.  Returns (ms per loop iteration, ms per do_step
+    alone), each over n iterations after a short warm-up."""
+    def one(readout):
+        s.do_step(dt0)
+        if readout:
+            s.diagnostics()
+            s.time()
+            s.get_light_curves()
+
+    out = []
+    for readout in (True, False):
+        for _ in range(10):
+            one(readout)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            one(readout)
+        s.synchronize()
+        out.append((time.perf_counter() - t0) / n * 1e3)
+    return out
 
 
 def ensemble_runs(S, n_runs=64, seed=4):
@@ -255,6 +284,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble (C4) measurement")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--driver-steps", type=int, default=200,
+                    help="iterations of the reference driver's per-step loop (do_step + readouts) to time")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -292,6 +323,7 @@ def main():
     tmx = smx.time()
     dt_eff = smx.last_dt()
     prof = profile(smx, args.dt, args.profile_steps)
+    loop_ms, step_ms = driver_loop(smx, args.dt, args.driver_steps)
     del smx
 
     l1 = float(np.sum(np.abs(Umx[:, 0] - U64[:, 0])) / np.sum(np.abs(U64[:, 0])))
@@ -344,6 +376,10 @@ def main():
                      "algorithmic_bytes": ab, "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "stop_check": bool(stop),
         "sim_time": [t64, tmx],
+        "driver_loop": {"steps": args.driver_steps, "ms_per_step": loop_ms, "do_step_ms": step_ms,
+                        "readout_ms": loop_ms - step_ms,
+                        "loop": "FP32-flux do_step, then diagnostics sums, time() and get_light_curves(), "
+                                "eager, after the timed region"},
     }
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(S, args.ensemble_steps, args.warmup, args.dt, args.profile_steps)
