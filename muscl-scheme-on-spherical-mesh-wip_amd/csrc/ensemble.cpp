@@ -86,7 +86,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     std::vector<double> fcn4, area, rate, U, light_w;
     std::vector<int2> seg;  // readout workgroups: each run's faces in blocks from its first face
     std::vector<int32_t> seg_run, run_part;
-    const int32_t kReadoutFaces = fp64::block_size();
+    const int32_t kReadoutFaces = fp64::readout_faces();
     std::vector<PhysParams> phys;
     int32_t F = 0, H = 0, halo = 0, fsets = 0, fset_max = 1, slot_max = 4;
     for (int r = 0; r < R; ++r) {

@@ -35,6 +35,7 @@ void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEven
 // into out[9] = {Sigma A, L A (3), E A, flux_0, flux_45, flux_90, flux_180}:
 // two launches, with per-block sums in partial[readout_blocks(nf)][9].
 // light_w: [nf][4] observer weights (dim 5; unused for dim 4).
+int readout_faces();  // faces per readout workgroup
 int readout_blocks(int nf);
 void enqueue_readout(const StepArgs& a, const double* light_w, double* partial, double* out, hipStream_t s);
 // The same for every run of an ensemble: n_part workgroups, seg[b] the face

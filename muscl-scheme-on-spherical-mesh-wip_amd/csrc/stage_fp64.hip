@@ -49,7 +49,8 @@ void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const Ker
 }
 
 
-int readout_blocks(int nf) { return (nf + kBlock - 1) / kBlock; }
+int readout_faces() { return kReadoutFaces; }
+int readout_blocks(int nf) { return (nf + kReadoutFaces - 1) / kReadoutFaces; }
 
 void enqueue_readout(const StepArgs& a, const double* light_w, double* partial, double* out, hipStream_t s) {
     enqueue_readout_runs(a, light_w, nullptr, nullptr, nullptr, readout_blocks(a.geom.nf), partial, out, s);
