@@ -185,7 +185,12 @@ void Engine::upload(const RunHost& r) {
 
     d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(fp64::readout_blocks(nf)), nullptr));
     d_readout_ = static_cast<double*>(dev(sizeof(double) * 9, nullptr));
-    if (dim_ == 5) d_light_w_ = static_cast<double*>(dev(sizeof(double) * r.light_w.size(), r.light_w.data()));
+    if (dim_ == 5) {
+        double obs[16];
+        light_observers(obs);
+        d_light_mask_ = static_cast<uint8_t*>(dev(r.light_mask.size(), r.light_mask.data()));
+        d_light_obs_ = static_cast<double*>(dev(sizeof(obs), obs));
+    }
     check(hipHostMalloc(reinterpret_cast<void**>(&h_readout_), sizeof(Readout), hipHostMallocDefault),
           "hipHostMalloc");
 
@@ -290,7 +295,7 @@ Scalars Engine::read_scalars() {
 
 void Engine::ensure_readout() {
     if (readout_valid_) return;
-    fp64::enqueue_readout(args_, d_light_w_, d_readout_part_, d_readout_, stream_);
+    fp64::enqueue_readout(args_, d_light_mask_, d_light_obs_, d_readout_part_, d_readout_, stream_);
     check(hipGetLastError(), "readout launch");
     check(hipMemcpyAsync(h_readout_->sums, d_readout_, sizeof(h_readout_->sums), hipMemcpyDeviceToHost, stream_),
           "read readout");

@@ -97,7 +97,8 @@ private:
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;
     StepArgs args_{};
-    double* d_light_w_ = nullptr;       // [nf][4] observer weights (dim 5)
+    uint8_t* d_light_mask_ = nullptr;   // [nf] light-curve observer visibility bits (dim 5)
+    double* d_light_obs_ = nullptr;     // [16] observer positions and norms
     double* d_readout_part_ = nullptr;  // [readout blocks][9] per-block sums
     double* d_readout_ = nullptr;       // [9] diagnostics sums, then light curves
     // pinned host copy of the last readout and of the scalars; readout_valid_

@@ -83,7 +83,8 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
     std::vector<float> slot_b, edge32, inv_area;
-    std::vector<double> fcn4, area, rate, U, light_w;
+    std::vector<double> fcn4, area, rate, U;
+    std::vector<uint8_t> light_mask;
     std::vector<int2> seg;  // readout workgroups: each run's faces in blocks from its first face
     std::vector<int32_t> seg_run, run_part;
     const int32_t kReadoutFaces = fp64::readout_faces();
@@ -115,7 +116,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         append(inv_area, m.inv_area);
         append(fcn4, rh.fcn4);
         append(area, rh.area);
-        if (dim == 5) append(light_w, rh.light_w);
+        if (dim == 5) append(light_mask, rh.light_mask);
         run_part.push_back(int32_t(seg.size()));
         for (int32_t f = 0; f < nf; f += kReadoutFaces) {
             seg.push_back(make_int2(F + f, F + std::min(nf, f + kReadoutFaces)));
@@ -196,7 +197,12 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d_seg_ = static_cast<const int2*>(dev(sizeof(int2) * seg.size(), seg.data()));
         d_seg_run_ = static_cast<const int*>(dev(sizeof(int32_t) * seg_run.size(), seg_run.data()));
         d_run_part_ = static_cast<const int*>(dev(sizeof(int32_t) * run_part.size(), run_part.data()));
-        if (dim == 5) d_light_w_ = static_cast<double*>(dev(sizeof(double) * light_w.size(), light_w.data()));
+        if (dim == 5) {
+            double obs[16];
+            light_observers(obs);
+            d_light_mask_ = static_cast<uint8_t*>(dev(light_mask.size(), light_mask.data()));
+            d_light_obs_ = static_cast<double*>(dev(sizeof(obs), obs));
+        }
         d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * seg.size(), nullptr));
         d_readout_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(R), nullptr));
         readout_.assign(9 * size_t(R), 0.0);
@@ -309,8 +315,8 @@ Scalars Ensemble::read_scalars(int r) {
 
 void Ensemble::ensure_readout() {
     if (readout_valid_) return;
-    fp64::enqueue_readout_runs(args_, d_light_w_, d_seg_, d_seg_run_, d_run_part_, n_seg_, d_readout_part_, d_readout_,
-                               stream_);
+    fp64::enqueue_readout_runs(args_, d_light_mask_, d_light_obs_, d_seg_, d_seg_run_, d_run_part_, n_seg_,
+                               d_readout_part_, d_readout_, stream_);
     check(hipGetLastError(), "readout launch");
     check(hipMemcpyAsync(readout_.data(), d_readout_, sizeof(double) * readout_.size(), hipMemcpyDeviceToHost,
                          stream_),

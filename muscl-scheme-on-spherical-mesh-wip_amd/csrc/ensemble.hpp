@@ -84,7 +84,8 @@ private:
     const int* d_seg_run_ = nullptr;
     const int* d_run_part_ = nullptr;
     int n_seg_ = 0;
-    double* d_light_w_ = nullptr;
+    uint8_t* d_light_mask_ = nullptr;
+    double* d_light_obs_ = nullptr;
     double* d_readout_part_ = nullptr;
     double* d_readout_ = nullptr;
     std::vector<double> readout_;

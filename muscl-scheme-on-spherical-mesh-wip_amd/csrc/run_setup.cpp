@@ -19,6 +19,16 @@ inline double norm3(double x, double y, double z) {
 
 }  // namespace
 
+void light_observers(double obs[16]) {
+    // observers at 9400 pc along x, (x+z), z and -z (adiabatic.hpp:228-333)
+    const double d = 9.4 * 3 * 1e19 / 15000, sq = std::sqrt(9.4 * 3 * 1e19 / 15000);
+    const double o[4][3] = {{d, 0, 0}, {sq, 0, sq}, {0, 0, d}, {0, 0, -d}};
+    for (int q = 0; q < 4; ++q) {
+        for (int c = 0; c < 3; ++c) obs[4 * q + c] = o[q][c];
+        obs[4 * q + 3] = norm3(o[q][0], o[q][1], o[q][2]);
+    }
+}
+
 RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
                        const RunParams& params, int32_t slots, int32_t max_faces, int threads) {
     if (solver < kHLLE || solver > kHLLCPLUS) throw std::invalid_argument("unknown solver");
@@ -135,13 +145,17 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces) {
     }
 
     // get_light_curves (adiabatic.hpp:228-333): each observer's flux is
-    // sum_f Pi_f * cos(alpha) * A_f over the faces it sees, so the geometric
-    // factor mask * |r.o| / |o| * A is precomputed per face and observer
-    // (observers at 9400 pc along x, (x+z), z and -z)
-    r.light_w.assign(dim == 5 ? 4 * size_t(nf) : 0, 0.0);
+    // sum_f Pi_f * cos(alpha) * A_f over the faces it sees.  Which faces an
+    // observer sees comes from the reference's atan2 / acos tests, so it is
+    // decided here once per face (bit k for observer k); the device forms the
+    // weight |r.o| / |o| * A itself (r.o summed over x, y, z in that order,
+    // without contraction), the value this loop would form on the host.
+    r.light_mask.assign(dim == 5 ? size_t(nf) : 0, 0);
     if (dim == 5) {
-        const double d = 9.4 * 3 * 1e19 / 15000, sq = std::sqrt(9.4 * 3 * 1e19 / 15000);
-        const double obs[4][3] = {{d, 0, 0}, {sq, 0, sq}, {0, 0, d}, {0, 0, -d}};
+        double ob[16];
+        light_observers(ob);
+        const double obs[4][3] = {{ob[0], ob[1], ob[2]}, {ob[4], ob[5], ob[6]}, {ob[8], ob[9], ob[10]},
+                                  {ob[12], ob[13], ob[14]}};
         for (int32_t k = 0; k < nf; ++k) {
             const int32_t f = L.perm[k];
             const double* c = &g.fc[3 * size_t(f)];
@@ -149,16 +163,12 @@ void relayout(RunHost& r, int32_t slots, int32_t max_faces) {
             const double* u = &r.fcn4[4 * size_t(k)];
             const double phi = std::atan2(c[1] / n, c[0] / n);
             const double theta = std::acos(c[2] / n);
-            auto cos_a = [&](int o) {
-                const double dv = obs[o][0] * u[0] + obs[o][1] * u[1] + obs[o][2] * u[2];
-                return std::abs(dv) / norm3(obs[o][0], obs[o][1], obs[o][2]);
-            };
-            const double A = r.area[k];
-            double* w = &r.light_w[4 * size_t(k)];
-            if (phi < M_PI / 2 && phi > -M_PI / 2) w[0] = cos_a(0) * A;
-            if (obs[1][0] * u[0] + obs[1][1] * u[1] + obs[1][2] * u[2] > 0) w[1] = cos_a(1) * A;
-            if (theta < M_PI / 2) w[2] = cos_a(2) * A;
-            if (theta > M_PI / 2) w[3] = cos_a(3) * A;
+            uint8_t m = 0;
+            if (phi < M_PI / 2 && phi > -M_PI / 2) m |= 1;
+            if (obs[1][0] * u[0] + obs[1][1] * u[1] + obs[1][2] * u[2] > 0) m |= 2;
+            if (theta < M_PI / 2) m |= 4;
+            if (theta > M_PI / 2) m |= 8;
+            r.light_mask[k] = m;
         }
     }
 

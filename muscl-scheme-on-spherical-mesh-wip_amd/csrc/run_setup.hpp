@@ -36,7 +36,7 @@ struct RunHost {
     std::vector<double> beta, hsc, ec, en;         // [nh][4] each
     // internal face order
     std::vector<double> area, fcn4, acc;           // [nf], [nf][4] (centre, round_diff / area), [nf][8]
-    std::vector<double> light_w;                   // [nf][4] (dim 5): observer weights of get_light_curves
+    std::vector<uint8_t> light_mask;               // [nf] (dim 5): bit k set when light-curve observer k sees the face
     PhysParams phys{};
     RunParams params;
 };
@@ -49,6 +49,11 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
 // Rebuilds the layout and every table in its order for another patch shape
 // (the geometry and physical parameters are kept).
 void relayout(RunHost& r, int32_t slots, int32_t max_faces);
+
+// The four light-curve observers of get_light_curves: obs[4 k .. 4 k + 2]
+// their positions, obs[4 k + 3] the norm; the readout forms a visible
+// face's weight as |r . o| / |o| * A (r the unit face centre, A the area).
+void light_observers(double obs[16]);
 
 // FP32 tables of the mixed kernel, per extended slot row (block b's own then
 // halo half-edges start at foff[f0] + halo_off[b]) and per edge of a block
