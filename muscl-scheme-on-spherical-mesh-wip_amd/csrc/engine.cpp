@@ -184,15 +184,17 @@ void Engine::upload(const RunHost& r) {
     args_.sc = static_cast<Scalars*>(dev(sizeof(Scalars), &s));
 
     d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(fp64::readout_blocks(nf)), nullptr));
-    d_readout_ = static_cast<double*>(dev(sizeof(double) * 9, nullptr));
     if (dim_ == 5) {
         double obs[16];
         light_observers(obs);
         d_light_mask_ = static_cast<uint8_t*>(dev(r.light_mask.size(), r.light_mask.data()));
         d_light_obs_ = static_cast<double*>(dev(sizeof(obs), obs));
     }
-    check(hipHostMalloc(reinterpret_cast<void**>(&h_readout_), sizeof(Readout), hipHostMallocDefault),
+    check(hipHostMalloc(reinterpret_cast<void**>(&h_readout_), sizeof(Readout),
+                        hipHostMallocMapped | hipHostMallocCoherent),
           "hipHostMalloc");
+    check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_readout_view_), h_readout_, 0),
+          "hipHostGetDevicePointer");
 
     perm_ = L.perm;
     he_old_ = L.he_old;
@@ -295,12 +297,9 @@ Scalars Engine::read_scalars() {
 
 void Engine::ensure_readout() {
     if (readout_valid_) return;
-    fp64::enqueue_readout(args_, d_light_mask_, d_light_obs_, d_readout_part_, d_readout_, stream_);
+    fp64::enqueue_readout(args_, d_light_mask_, d_light_obs_, d_readout_part_, d_readout_view_->sums,
+                          &d_readout_view_->sc, stream_);
     check(hipGetLastError(), "readout launch");
-    check(hipMemcpyAsync(h_readout_->sums, d_readout_, sizeof(h_readout_->sums), hipMemcpyDeviceToHost, stream_),
-          "read readout");
-    check(hipMemcpyAsync(&h_readout_->sc, args_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_),
-          "read scalars");
     synchronize();
     readout_valid_ = true;
 }

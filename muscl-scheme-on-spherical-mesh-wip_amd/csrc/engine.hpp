@@ -100,14 +100,15 @@ private:
     uint8_t* d_light_mask_ = nullptr;   // [nf] light-curve observer visibility bits (dim 5)
     double* d_light_obs_ = nullptr;     // [16] observer positions and norms
     double* d_readout_part_ = nullptr;  // [readout blocks][9] per-block sums
-    double* d_readout_ = nullptr;       // [9] diagnostics sums, then light curves
-    // pinned host copy of the last readout and of the scalars; readout_valid_
+    // mapped pinned host memory that the readout kernel writes: the
+    // diagnostics sums, the light curves and the scalars; readout_valid_
     // holds until the next launch or upload that changes the state
     struct Readout {
         double sums[9];
         Scalars sc;
     };
     Readout* h_readout_ = nullptr;
+    Readout* d_readout_view_ = nullptr;  // its device address
     bool readout_valid_ = false;
     bool stepped_ = false;              // a step ran since the state was set
     double* d_curl_ = nullptr;          // lazily: ev [nh][3], out [nf], W_tmp [nf][dim]

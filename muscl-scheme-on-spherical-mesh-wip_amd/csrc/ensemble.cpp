@@ -204,9 +204,15 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
             d_light_obs_ = static_cast<double*>(dev(sizeof(obs), obs));
         }
         d_readout_part_ = static_cast<double*>(dev(sizeof(double) * 9 * seg.size(), nullptr));
-        d_readout_ = static_cast<double*>(dev(sizeof(double) * 9 * size_t(R), nullptr));
-        readout_.assign(9 * size_t(R), 0.0);
-        scalars_.assign(size_t(R), Scalars{});
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_sums_), sizeof(double) * 9 * size_t(R),
+                            hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_scalars_), sizeof(Scalars) * size_t(R),
+                            hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc");
+        check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_sums_view_), h_sums_, 0), "hipHostGetDevicePointer");
+        check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_scalars_view_), h_scalars_, 0),
+              "hipHostGetDevicePointer");
         const size_t row = dim == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
@@ -216,6 +222,8 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {
         for (void* p : allocations_) (void)hipFree(p);
+        if (h_sums_) (void)hipHostFree(h_sums_);
+        if (h_scalars_) (void)hipHostFree(h_scalars_);
         (void)hipStreamDestroy(stream_);
         throw;
     }
@@ -226,6 +234,8 @@ Ensemble::~Ensemble() {
     if (graph_) (void)hipGraphDestroy(graph_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
+    if (h_sums_) (void)hipHostFree(h_sums_);
+    if (h_scalars_) (void)hipHostFree(h_scalars_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -306,7 +316,7 @@ void Ensemble::synchronize() { check(hipStreamSynchronize(stream_), "hipStreamSy
 
 Scalars Ensemble::read_scalars(int r) {
     if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
-    if (readout_valid_) return scalars_[size_t(r)];
+    if (readout_valid_) return h_scalars_[r];
     Scalars s;
     check(hipMemcpyAsync(&s, args_.sc + r, sizeof(Scalars), hipMemcpyDeviceToHost, stream_), "read scalars");
     synchronize();
@@ -316,14 +326,8 @@ Scalars Ensemble::read_scalars(int r) {
 void Ensemble::ensure_readout() {
     if (readout_valid_) return;
     fp64::enqueue_readout_runs(args_, d_light_mask_, d_light_obs_, d_seg_, d_seg_run_, d_run_part_, n_seg_,
-                               d_readout_part_, d_readout_, stream_);
+                               d_readout_part_, d_sums_view_, d_scalars_view_, stream_);
     check(hipGetLastError(), "readout launch");
-    check(hipMemcpyAsync(readout_.data(), d_readout_, sizeof(double) * readout_.size(), hipMemcpyDeviceToHost,
-                         stream_),
-          "read readout");
-    check(hipMemcpyAsync(scalars_.data(), args_.sc, sizeof(Scalars) * scalars_.size(), hipMemcpyDeviceToHost,
-                         stream_),
-          "read scalars");
     synchronize();
     readout_valid_ = true;
 }
@@ -331,14 +335,14 @@ void Ensemble::ensure_readout() {
 void Ensemble::diagnostics(int r, double out[5]) {
     if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
     ensure_readout();
-    for (int k = 0; k < 5; ++k) out[k] = readout_[9 * size_t(r) + k];
+    for (int k = 0; k < 5; ++k) out[k] = h_sums_[9 * size_t(r) + k];
 }
 
 void Ensemble::light_curves(int r, double out[4]) {
     if (r < 0 || r >= n_runs()) throw std::out_of_range("run index");
     if (dim_ != 5) throw std::invalid_argument("light curves exist for the adiabatic (dim 5) solvers only");
     ensure_readout();
-    for (int k = 0; k < 4; ++k) out[k] = readout_[9 * size_t(r) + 5 + k];
+    for (int k = 0; k < 4; ++k) out[k] = h_sums_[9 * size_t(r) + 5 + k];
 }
 
 double Ensemble::time(int r) { return read_scalars(r).t; }

@@ -87,9 +87,12 @@ private:
     uint8_t* d_light_mask_ = nullptr;
     double* d_light_obs_ = nullptr;
     double* d_readout_part_ = nullptr;
-    double* d_readout_ = nullptr;
-    std::vector<double> readout_;
-    std::vector<Scalars> scalars_;
+    // mapped pinned host memory the readout kernel writes ([n_runs][9] sums,
+    // [n_runs] scalars), and its device addresses
+    double* h_sums_ = nullptr;
+    Scalars* h_scalars_ = nullptr;
+    double* d_sums_view_ = nullptr;
+    Scalars* d_scalars_view_ = nullptr;
     bool readout_valid_ = false;
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;

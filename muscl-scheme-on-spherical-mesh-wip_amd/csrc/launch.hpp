@@ -32,20 +32,23 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_finalize_dt(const StepArgs& a, double dt0, hipStream_t s, const KernelEvents* ev = nullptr);
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
 // The diagnostics sums and (dim 5) the light curves of the current state
-// into out[9] = {Sigma A, L A (3), E A, flux_0, flux_45, flux_90, flux_180}:
-// two launches, with per-block sums in partial[readout_blocks(nf)][9].
+// into out[9] = {Sigma A, L A (3), E A, flux_0, flux_45, flux_90, flux_180},
+// and a copy of the scalars into sc_out[1] (both device views of mapped host
+// memory): two launches, with per-block sums in
+// partial[readout_blocks(nf)][9].
 // light_mask [nf] / light_obs [16]: the light-curve observers' visibility
 // bits and positions (dim 5; unused for dim 4).
 int readout_faces();  // faces per readout workgroup
 int readout_blocks(int nf);
 void enqueue_readout(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, double* partial,
-                     double* out, hipStream_t s);
+                     double* out, Scalars* sc_out, hipStream_t s);
 // The same for every run of an ensemble: n_part workgroups, seg[b] the face
 // range and seg_run[b] the run of workgroup b, run_part[r] the first
-// workgroup of run r (run_part[n_runs] = n_part); out[n_runs][9].
+// workgroup of run r (run_part[n_runs] = n_part); out[n_runs][9],
+// sc_out[n_runs].
 void enqueue_readout_runs(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, const int2* seg,
                           const int* seg_run, const int* run_part, int n_part, double* partial, double* out,
-                          hipStream_t s);
+                          Scalars* sc_out, hipStream_t s);
 // write_t_curl per face [nf] from the last stage-2 input (W2, or W32b widened
 // into W_tmp [nf][dim]); stepped = false gives the value before any step.
 // ev = v_j - v_j1 per half-edge [nh][3].

@@ -53,20 +53,20 @@ int readout_faces() { return kReadoutFaces; }
 int readout_blocks(int nf) { return (nf + kReadoutFaces - 1) / kReadoutFaces; }
 
 void enqueue_readout(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, double* partial,
-                     double* out, hipStream_t s) {
+                     double* out, Scalars* sc_out, hipStream_t s) {
     enqueue_readout_runs(a, light_mask, light_obs, nullptr, nullptr, nullptr, readout_blocks(a.geom.nf), partial, out,
-                         s);
+                         sc_out, s);
 }
 
 void enqueue_readout_runs(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, const int2* seg,
                           const int* seg_run, const int* run_part, int n_part, double* partial, double* out,
-                          hipStream_t s) {
+                          Scalars* sc_out, hipStream_t s) {
     if (a.phys.dim == 5)
         k_readout_partial<5><<<n_part, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, light_mask, light_obs, seg, seg_run,
                                                        partial);
     else k_readout_partial<4><<<n_part, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, light_mask, light_obs, seg, seg_run,
                                                          partial);
-    k_readout_final<<<run_part ? a.geom.n_runs : 1, kBlock, 0, s>>>(partial, run_part, n_part, out);
+    k_readout_final<<<run_part ? a.geom.n_runs : 1, kBlock, 0, s>>>(partial, run_part, n_part, out, a.sc, sc_out);
 }
 
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s) {
