@@ -164,6 +164,35 @@ def test_c2_full_size_fp64_parity_and_conservation():
     assert np.allclose(d, d0, rtol=1e-13, atol=1e-15)
 
 
+def test_c3_full_size_fp64_parity_and_mixed_error():
+    """quad_sphere(9) (1,572,864 faces) with the default benchmark's physics:
+    HLLC, omega_ns = 0.1, accretion on (rate 1e-3, e_acc 0.1, omega_acc 0.8,
+    tilt 0.3, width 0.1) and friction with alpha = 1e-6.  The FP64 path
+    matches the oracle to 1e-10 after 10 steps, sources included, and the
+    FP32-flux path stays within 1e-5 relative L1 density error of the FP64
+    path after 1000 steps."""
+    import splash_amd as S
+    p = S.default_params(setup_threads=16, accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1,
+                         omega_acc_abs=0.8, tilt_angle=0.3, acc_width=0.1, alpha=1e-6)
+    mesh = S.quad_sphere(9)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=2, sigma_amp=0.1, pi_frac=0.1, omega=(0.0, 0.0, 0.5),
+                              noise=0.01)
+    gpu, ref = _pair(S, "quad", (9,), "HLLC", U0, 10, 1e-4, omega_ns=0.1, params=p, threads=16)
+    err = _rel(gpu.get_state(), ref.state())
+    print(f"C3 quad(9) 10 steps: FP64 max rel err vs oracle {err:.3e}")
+    assert err <= TOL
+    assert gpu.time() == pytest.approx(ref.time(), rel=1e-12)
+    del gpu, ref
+    f64 = S.MUSCL_HLLC(mesh, U0, 5, GAM, 0.1, True, params=p, precision=S.FP64)
+    mx = S.MUSCL_HLLC(mesh, U0, 5, GAM, 0.1, True, params=p, precision=S.MIXED)
+    f64.do_steps(1e-4, 1000)
+    mx.do_steps(1e-4, 1000)
+    A, B = f64.get_state(), mx.get_state()
+    l1 = np.sum(np.abs(B[:, 0] - A[:, 0])) / np.sum(np.abs(A[:, 0]))
+    print(f"C3 quad(9) 1000 steps: FP32-flux vs FP64 L1 rel density err {l1:.3e}")
+    assert l1 <= 1e-5
+
+
 def test_c5b_full_size_mixed_vs_fp64_and_conservation():
     """icosphere_hex(9) (2,621,442 faces, 15.7M half-edges), the largest
     single-mesh workload: after 100 steps of HLLC+ with sources off, the
