@@ -193,6 +193,38 @@ def test_c3_full_size_fp64_parity_and_mixed_error():
     assert l1 <= 1e-5
 
 
+def test_c5a_full_size_fp64_parity_and_mixed_error():
+    """uv_sphere(2048, 1024) (2,097,152 faces: quads plus the pole triangle
+    fans), HLLC+, omega_ns = 0.1, accretion and friction as the benchmark, a
+    spherical-harmonic density at rest: the FP64 path matches the oracle to
+    1e-10 after 5 steps, and the FP32-flux path stays within 1e-5 relative L1
+    density error of the FP64 path after 1000 steps.  (The flow starts at
+    rest, so after 5 steps the momenta peak at ~1e-7: their last-bit
+    differences, ~1e-20 absolute, normalise to ~3e-12; density, energy and
+    dt agree exactly.)"""
+    import splash_amd as S
+    p = S.default_params(setup_threads=16, accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1,
+                         omega_acc_abs=0.8, tilt_angle=0.3, acc_width=0.1, alpha=1e-6)
+    mesh = S.uv_sphere(2048, 1024)
+    fc = mesh.face_centres()
+    sigma = 1.0 + S.ic.real_sph_harm_sum(fc, seed=5, n_modes=8, lmax=16, amplitude=0.05)
+    U0 = S.ic.adiabatic_state(fc, GAM, seed=5, sigma=sigma, pressure=np.full_like(sigma, 0.1), noise=0.0)
+    gpu, ref = _pair(S, "uv", (2048, 1024), "HLLCplus", U0, 5, 1e-4, omega_ns=0.1, params=p, threads=16)
+    err = _rel(gpu.get_state(), ref.state())
+    print(f"C5a uv(2048,1024) 5 steps: FP64 max rel err vs oracle {err:.3e}, dt {gpu.last_dt():.3e}")
+    assert err <= TOL
+    assert gpu.time() == pytest.approx(ref.time(), rel=1e-12)
+    del gpu, ref
+    f64 = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.1, True, params=p, precision=S.FP64)
+    mx = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.1, True, params=p, precision=S.MIXED)
+    f64.do_steps(1e-4, 1000)
+    mx.do_steps(1e-4, 1000)
+    A, B = f64.get_state(), mx.get_state()
+    l1 = np.sum(np.abs(B[:, 0] - A[:, 0])) / np.sum(np.abs(A[:, 0]))
+    print(f"C5a uv(2048,1024) 1000 steps: FP32-flux vs FP64 L1 rel density err {l1:.3e}")
+    assert l1 <= 1e-5
+
+
 def test_c5b_full_size_mixed_vs_fp64_and_conservation():
     """icosphere_hex(9) (2,621,442 faces, 15.7M half-edges), the largest
     single-mesh workload: after 100 steps of HLLC+ with sources off, the
