@@ -4,6 +4,8 @@ import ctypes as C
 import glob
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 
@@ -29,6 +31,20 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in sorted(declared) if not hasattr(L, n)]
     assert not missing, missing
     assert declared <= set(_lib.SIGNATURES), declared - set(_lib.SIGNATURES)
+
+
+def test_c_header_is_plain_c():
+    """include/splash.h is the boundary any FFI binds: it compiles as strict
+    C99 and C11 with every warning an error."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no C compiler")
+    header = os.path.join(ROOT, "include", "splash.h")
+    for std in ("c99", "c11"):
+        r = subprocess.run([cc, "-x", "c", f"-std={std}", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                            header], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
 
 
 def test_host_mesh_and_geometry_are_bit_identical_to_the_oracle():
