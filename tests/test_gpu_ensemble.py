@@ -95,6 +95,29 @@ def test_ensemble_set_state_resumes_exactly():
         assert a.get_stop_check(r) == b.get_stop_check(r)
 
 
+def test_ensemble_of_many_small_runs():
+    """120 runs of 42 to 642 faces (icosphere_hex levels 1-3, a few patches
+    each, many sharing one 8-block padding group): every run's state, time
+    and light curves equal those of its single solver bit for bit."""
+    import splash_amd as S
+    meshes = [S.icosphere_hex(1 + k % 3) for k in range(3)]
+    runs = []
+    for k in range(120):
+        m = meshes[k % 3]
+        U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=100 + k, omega=(0.0, 0.1 * (k % 5), 0.3))
+        runs.append((m, U0, GAM - 0.001 * (k % 7), 0.0, None))
+    ens = S.Ensemble(runs, 5, S.HLLC)
+    dt0 = [1e-4 * (1 + k % 4) for k in range(len(runs))]
+    ens.do_steps(dt0, 21)
+    for r in range(0, len(runs), 7):
+        m, U0, g, om, p = runs[r]
+        s = S.MUSCL_HLLC(m, U0, 5, g, om, False, params=p, precision=S.MIXED)
+        s.do_steps(dt0[r], 21)
+        assert np.array_equal(ens.get_state(r), s.get_state())
+        assert ens.time(r) == s.time()
+        assert np.array_equal(ens.get_light_curves(r), s.get_light_curves())
+
+
 def test_ensemble_isothermal_and_changing_dt0():
     """dim 4 / MUSCL_HLLE runs, and a per-run dt0 that changes between calls."""
     import splash_amd as S
