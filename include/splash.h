@@ -76,7 +76,8 @@ typedef struct splash_params {
     double acc_width;
     double alpha;
     int32_t precision; /* SPLASH_FP64 or SPLASH_MIXED */
-    int32_t setup_threads; /* host threads for the geometry precompute, 0 = 1 */
+    int32_t setup_threads; /* host threads for the geometry precompute; 0 = every host thread
+                            * (OpenMP's default team); the tables do not depend on it */
     int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..64; 0 = 64;
                           * -1 = the fastest of a few candidates, measured at creation */
 } splash_params;

@@ -3,6 +3,8 @@
 // OpenMP, so the tables do not depend on the thread count.
 #include "geometry.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -65,7 +67,7 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
     const auto& FV = mesh.fv;
     auto vert = [&](int32_t v) { return ld(&X[3 * size_t(v)]); };
     auto val = [&](int32_t f) { return mesh.foff[f + 1] - mesh.foff[f]; };
-    if (threads <= 0) threads = 1;
+    if (threads <= 0) threads = omp_get_max_threads();
 
     // vertex -> incident faces (CSR); the reference sorts and de-duplicates
     // the gathered list (:91-118), so only the set of faces matters.
@@ -324,8 +326,11 @@ Geometry build_geometry(const PolyMesh& mesh, int threads) {
     }
     if (missing) throw std::runtime_error("stencil search found no neighbour for " + std::to_string(missing) + " half-edges");
 
+    // shortest edge arc (MUSCL_base.hpp:63-91): a minimum, so the thread
+    // count cannot change it
     double h0 = 10;
     int32_t maxval = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(min : h0) reduction(max : maxval)
     for (int32_t f = 0; f < nf; ++f) {
         const int32_t* F = &FV[mesh.foff[f]];
         const int32_t n = val(f);

@@ -46,6 +46,8 @@ struct Geometry {
 
 // Throws std::runtime_error when the mesh is not a closed manifold whose
 // every edge has exactly one neighbour (the reference's assumption).
+// threads <= 0: OpenMP's default team (every host thread unless
+// OMP_NUM_THREADS says otherwise); the tables do not depend on it
 Geometry build_geometry(const PolyMesh& mesh, int threads = 0);
 
 }  // namespace splash

@@ -71,8 +71,11 @@ void bisect(std::vector<int32_t>& p, size_t lo, size_t hi, const std::vector<dou
         const double cx = c[3 * size_t(x) + ax], cy = c[3 * size_t(y) + ax];
         return cx < cy || (cx == cy && x < y);
     });
+    // the halves are disjoint ranges of p: large ones split into tasks
+#pragma omp task default(shared) if (half > 65536)
     bisect(p, lo, lo + half, c, leaf);
     bisect(p, lo + half, hi, c, leaf);
+#pragma omp taskwait
 }
 
 
@@ -142,6 +145,8 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     if (try_bisection) {
         L.perm.resize(nf);
         for (int32_t f = 0; f < nf; ++f) L.perm[f] = f;
+#pragma omp parallel
+#pragma omp single
         bisect(L.perm, 0, size_t(nf), g.fcn, size_t(max_faces));
         L.iperm = inverse(L.perm);
         L.blk_face = pack(g, L.perm, L.iperm, slots, max_faces, max_faces);
@@ -247,57 +252,67 @@ void validate_layout(const Geometry& g, const Layout& L) {
     require(nb % 8 == 0 && L.blk_face[0] == 0 && L.blk_face[nb] == nf, "blocks cover the faces");
     require(int32_t(L.halo_off.size()) == nb + 1 && L.halo_off[nb] == int32_t(L.halo_he.size()), "halo offsets");
     require(int32_t(L.fset_off.size()) == nb + 1 && L.n_ext() == nh + int32_t(L.halo_he.size()), "face sets");
+    // the per-block checks are independent: they run in parallel, and the
+    // first failure (in no particular order) is rethrown after the loop
+    std::string failure;
+#pragma omp parallel for schedule(dynamic, 64)
     for (int32_t b = 0; b < nb; ++b) {
-        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
-        require(f0 <= f1 && f1 - f0 <= L.max_faces, "faces per block within the limit");
-        const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
-        const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
-        require(n_halo >= 0 && n_own + n_halo <= L.slots, "own + halo half-edges fit the LDS slots");
-        for (int32_t q = L.halo_off[b]; q < L.halo_off[b + 1]; ++q) {
-            const int32_t he = L.halo_he[q];
-            require(he >= 0 && he < nh && (he < h0 || he >= h1), "halo half-edges are outside the block");
-        }
-        // one-sided fluxes of the mixed kernel: an own half-edge evaluates its
-        // edge when it is the side of lower mesh index or its twin is in the
-        // halo, and also fills its twin's slot when that twin is own; every
-        // own slot must be written exactly once
-        std::vector<int32_t> filled(size_t(n_own), 0);
-        for (int32_t hn = h0; hn < h1; ++hn) {
-            const int32_t tw = L.twin_slot[2 * size_t(hn)];
-            const bool twin_own = tw >= h0 && tw < h1;
-            const bool low = L.he_old[hn] < L.he_old[tw];
-            if (low || !twin_own) filled[size_t(hn - h0)]++;
-            if (low && twin_own) filled[size_t(tw - h0)]++;
-        }
-        for (int32_t q = 0; q < n_own; ++q) require(filled[size_t(q)] == 1, "each own flux written once");
-        // face set: own faces first, every local index in range and pointing
-        // at the face the geometry names
-        const int32_t fs0 = L.fset_off[b], n_fs = L.fset_off[b + 1] - fs0;
-        require(n_fs >= f1 - f0 && n_fs <= Layout::kFaceSetMax, "face set size");
-        for (int32_t f = f0; f < f1; ++f) require(L.fset_face[fs0 + f - f0] == f, "own faces lead the face set");
-        for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
-            const int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
-            const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
-            require(row[3] == he, "extended row names its half-edge");
-            const int32_t h = L.he_old[he];
-            const int32_t want[5] = {L.iperm[g.he_face[h]], L.iperm[g.ff_plus[2 * size_t(h)]],
-                                     L.iperm[g.ff_plus[2 * size_t(h) + 1]], L.iperm[g.ff_minus[2 * size_t(h)]],
-                                     L.iperm[g.ff_minus[2 * size_t(h) + 1]]};
-            const int32_t got[5] = {row[0] & 0xffff, row[1] & 0xffff, row[1] >> 16, row[2] & 0xffff, row[2] >> 16};
-            for (int q = 0; q < 5; ++q) {
-                require(got[q] >= 0 && got[q] < n_fs, "local face index in range");
-                require(L.fset_face[fs0 + got[q]] == want[q], "local face index names the stencil face");
+        try {
+            const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+            require(f0 <= f1 && f1 - f0 <= L.max_faces, "faces per block within the limit");
+            const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
+            const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
+            require(n_halo >= 0 && n_own + n_halo <= L.slots, "own + halo half-edges fit the LDS slots");
+            for (int32_t q = L.halo_off[b]; q < L.halo_off[b + 1]; ++q) {
+                const int32_t he = L.halo_he[q];
+                require(he >= 0 && he < nh && (he < h0 || he >= h1), "halo half-edges are outside the block");
             }
-            require((row[0] >> 16) == g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]], "valence");
-        }
-        for (int32_t hn = h0; hn < h1; ++hn) {
-            const int32_t tw = L.twin_slot[2 * size_t(hn)], slot = L.twin_slot[2 * size_t(hn) + 1];
-            require(tw >= 0 && tw < nh && L.twin_slot[2 * size_t(tw)] == hn, "twin is an involution");
-            require(slot >= 0 && slot < n_own + n_halo, "twin slot in range");
-            const int32_t at = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
-            require(at == tw, "twin slot holds the twin");
+            // one-sided fluxes of the mixed kernel: an own half-edge evaluates its
+            // edge when it is the side of lower mesh index or its twin is in the
+            // halo, and also fills its twin's slot when that twin is own; every
+            // own slot must be written exactly once
+            std::vector<int32_t> filled(size_t(n_own), 0);
+            for (int32_t hn = h0; hn < h1; ++hn) {
+                const int32_t tw = L.twin_slot[2 * size_t(hn)];
+                const bool twin_own = tw >= h0 && tw < h1;
+                const bool low = L.he_old[hn] < L.he_old[tw];
+                if (low || !twin_own) filled[size_t(hn - h0)]++;
+                if (low && twin_own) filled[size_t(tw - h0)]++;
+            }
+            for (int32_t q = 0; q < n_own; ++q) require(filled[size_t(q)] == 1, "each own flux written once");
+            // face set: own faces first, every local index in range and pointing
+            // at the face the geometry names
+            const int32_t fs0 = L.fset_off[b], n_fs = L.fset_off[b + 1] - fs0;
+            require(n_fs >= f1 - f0 && n_fs <= Layout::kFaceSetMax, "face set size");
+            for (int32_t f = f0; f < f1; ++f) require(L.fset_face[fs0 + f - f0] == f, "own faces lead the face set");
+            for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
+                const int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
+                const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+                require(row[3] == he, "extended row names its half-edge");
+                const int32_t h = L.he_old[he];
+                const int32_t want[5] = {L.iperm[g.he_face[h]], L.iperm[g.ff_plus[2 * size_t(h)]],
+                                         L.iperm[g.ff_plus[2 * size_t(h) + 1]], L.iperm[g.ff_minus[2 * size_t(h)]],
+                                         L.iperm[g.ff_minus[2 * size_t(h) + 1]]};
+                const int32_t got[5] = {row[0] & 0xffff, row[1] & 0xffff, row[1] >> 16, row[2] & 0xffff, row[2] >> 16};
+                for (int q = 0; q < 5; ++q) {
+                    require(got[q] >= 0 && got[q] < n_fs, "local face index in range");
+                    require(L.fset_face[fs0 + got[q]] == want[q], "local face index names the stencil face");
+                }
+                require((row[0] >> 16) == g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]], "valence");
+            }
+            for (int32_t hn = h0; hn < h1; ++hn) {
+                const int32_t tw = L.twin_slot[2 * size_t(hn)], slot = L.twin_slot[2 * size_t(hn) + 1];
+                require(tw >= 0 && tw < nh && L.twin_slot[2 * size_t(tw)] == hn, "twin is an involution");
+                require(slot >= 0 && slot < n_own + n_halo, "twin slot in range");
+                const int32_t at = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+                require(at == tw, "twin slot holds the twin");
+            }
+        } catch (const std::exception& e) {
+#pragma omp critical(validate_layout_failure)
+            if (failure.empty()) failure = e.what();
         }
     }
+    if (!failure.empty()) throw std::logic_error(failure);
 }
 
 }  // namespace splash

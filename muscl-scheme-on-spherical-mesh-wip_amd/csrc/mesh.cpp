@@ -76,9 +76,15 @@ void HalfedgeMesh::adjust_outgoing(int v) {
 
 // surface_mesh.cpp:244-423
 int HalfedgeMesh::add_face(const int* vs, int n) {
-    std::vector<int> he(n), fresh(n), adjust(n, 0);
-    std::vector<std::pair<int, int>> links;
-    links.reserve(6 * n);
+    std::vector<int>& he = af_he_;
+    std::vector<int>& fresh = af_fresh_;
+    std::vector<int>& adjust = af_adjust_;
+    std::vector<std::pair<int, int>>& links = af_links_;
+    he.assign(size_t(n), 0);
+    fresh.assign(size_t(n), 0);
+    adjust.assign(size_t(n), 0);
+    links.clear();
+    links.reserve(6 * size_t(n));
     for (int i = 0, ii = 1; i < n; ++i, ii = (ii + 1) % n) {
         if (!boundary_v(vs[i])) throw std::runtime_error("add_face: complex vertex");
         he[i] = find_halfedge(vs[i], vs[ii]);
@@ -225,9 +231,17 @@ HalfedgeMesh::Vec HalfedgeMesh::centroid(int f) const {
     return c;
 }
 
+void HalfedgeMesh::reserve(int nv, int ne, int nf) {
+    pos_.reserve(size_t(nv));
+    v_he_.reserve(size_t(nv));
+    for (std::vector<int>* a : {&h_to_, &h_face_, &h_next_, &h_prev_}) a->reserve(2 * size_t(ne));
+    f_he_.reserve(size_t(nf));
+}
+
 // subdivision.cpp:180-342 (closed mesh without feature edges)
 void HalfedgeMesh::loop_subdivision() {
     const int nv = n_vertices(), ne = n_halfedges() / 2, nf = n_faces();
+    reserve(nv + ne, 2 * ne + 3 * nf, 4 * nf);  // each edge split, three new edges per triangle
     std::vector<Vec> vpoint(nv), epoint(ne);
     for (int v = 0; v < nv; ++v) {
         Vec p{0, 0, 0};
@@ -285,6 +299,7 @@ void HalfedgeMesh::loop_subdivision() {
 // subdivision.cpp:9-178 (closed mesh without feature edges)
 void HalfedgeMesh::catmull_clark_subdivision() {
     const int nv = n_vertices(), ne = n_halfedges() / 2, nf = n_faces();
+    reserve(nv + ne + nf, 4 * ne, 2 * ne);  // closed mesh: the faces' corners number 2 ne
     std::vector<Vec> vpoint(nv), epoint(ne), fpoint(nf);
     for (int f = 0; f < nf; ++f) fpoint[f] = centroid(f);
     for (int e = 0; e < ne; ++e) {
@@ -359,6 +374,7 @@ void HalfedgeMesh::catmull_clark_subdivision() {
 // the order of pmp's counter-clockwise face circulator, then assign.
 void HalfedgeMesh::dual() {
     HalfedgeMesh out;
+    out.reserve(n_faces(), n_halfedges() / 2, n_vertices());
     std::vector<int> fvertex(n_faces());
     for (int f = 0; f < n_faces(); ++f) {
         Vec c = centroid(f);

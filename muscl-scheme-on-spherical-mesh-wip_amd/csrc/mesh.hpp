@@ -45,6 +45,9 @@ public:
     int n_halfedges() const { return int(h_to_.size()); }
 
 private:
+    // capacity for nv vertices, ne edges and nf faces, so that growing to a
+    // known size does not copy the arrays repeatedly
+    void reserve(int nv, int ne, int nf);
     struct Vec {
         double x, y, z;
     };
@@ -67,6 +70,10 @@ private:
     std::vector<int> v_he_;
     std::vector<Vec> pos_;
     std::vector<int> f_he_;
+    // add_face's per-call work lists, kept between calls so that building a
+    // mesh of millions of faces does not allocate per face
+    std::vector<int> af_he_, af_fresh_, af_adjust_;
+    std::vector<std::pair<int, int>> af_links_;
 };
 
 // Generators (sph_gen.h:215-295); all return pmp-identical numbering.
