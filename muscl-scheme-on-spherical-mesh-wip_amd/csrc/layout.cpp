@@ -180,9 +180,26 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
     }
     auto twin_owner = [&](int32_t hn) { return L.iperm[g.he_face[g.twin[L.he_old[hn]]]]; };
 
+    // halo (the outside twins of each block's boundary half-edges, in the
+    // block's half-edge order) and the twin slots; the blocks are
+    // independent, so one parallel pass counts, a prefix sum places each
+    // block's halo, and a second parallel pass fills it
     const int32_t nb = L.n_blocks();
     L.twin_slot.assign(2 * size_t(nh), 0);
-    L.halo_off.assign(1, 0);
+    L.halo_off.assign(size_t(nb) + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+        int32_t n_halo = 0;
+        for (int32_t hn = L.foff[f0]; hn < L.foff[f1]; ++hn) {
+            const int32_t o = twin_owner(hn);
+            n_halo += (o < f0 || o >= f1) ? 1 : 0;
+        }
+        L.halo_off[size_t(b) + 1] = n_halo;
+    }
+    for (int32_t b = 0; b < nb; ++b) L.halo_off[size_t(b) + 1] += L.halo_off[size_t(b)];
+    L.halo_he.assign(size_t(L.halo_off[size_t(nb)]), 0);
+#pragma omp parallel for schedule(dynamic, 64)
     for (int32_t b = 0; b < nb; ++b) {
         const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
         const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
@@ -194,47 +211,59 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
             if (o >= f0 && o < f1) {
                 L.twin_slot[2 * size_t(hn) + 1] = tw - h0;
             } else {
-                L.twin_slot[2 * size_t(hn) + 1] = (h1 - h0) + n_halo++;
-                L.halo_he.push_back(tw);
+                L.twin_slot[2 * size_t(hn) + 1] = (h1 - h0) + n_halo;
+                L.halo_he[size_t(L.halo_off[size_t(b)] + n_halo++)] = tw;
             }
         }
-        L.halo_off.push_back(int32_t(L.halo_he.size()));
     }
 
-    // face sets and extended slot rows
+    // face sets and extended slot rows: each block gathers its set in its
+    // own list (own faces first, then stencil faces in slot order), with a
+    // per-thread face -> local index map that it clears again; the lists
+    // are then concatenated in block order
     const int32_t n_ext = nh + int32_t(L.halo_he.size());
     L.ext_loc.assign(4 * size_t(n_ext), 0);
-    L.fset_off.assign(1, 0);
-    std::vector<int32_t> local(nf, -1);
-    for (int32_t b = 0; b < nb; ++b) {
-        const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
-        const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
-        const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
-        const size_t first = L.fset_face.size();
-        auto loc = [&](int32_t f) {
-            if (local[f] < 0) {
-                local[f] = int32_t(L.fset_face.size() - first);
-                L.fset_face.push_back(f);
+    std::vector<std::vector<int32_t>> sets(static_cast<size_t>(nb));
+    int too_big = 0;
+#pragma omp parallel reduction(+ : too_big)
+    {
+        std::vector<int32_t> local(nf, -1);
+#pragma omp for schedule(dynamic, 64)
+        for (int32_t b = 0; b < nb; ++b) {
+            const int32_t f0 = L.blk_face[b], f1 = L.blk_face[b + 1];
+            const int32_t h0 = L.foff[f0], h1 = L.foff[f1];
+            const int32_t n_own = h1 - h0, n_halo = L.halo_off[b + 1] - L.halo_off[b];
+            std::vector<int32_t>& fs = sets[size_t(b)];
+            auto loc = [&](int32_t f) {
+                if (local[f] < 0) {
+                    local[f] = int32_t(fs.size());
+                    fs.push_back(f);
+                }
+                return local[f];
+            };
+            for (int32_t f = f0; f < f1; ++f) loc(f);
+            for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
+                const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
+                const int32_t h = L.he_old[he];
+                const int32_t owner = L.iperm[g.he_face[h]];
+                const int32_t val = g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]];
+                int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
+                row[0] = loc(owner) | (val << 16);
+                row[1] = loc(L.iperm[g.ff_plus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_plus[2 * size_t(h) + 1]]) << 16);
+                row[2] = loc(L.iperm[g.ff_minus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_minus[2 * size_t(h) + 1]]) << 16);
+                row[3] = he;
             }
-            return local[f];
-        };
-        for (int32_t f = f0; f < f1; ++f) loc(f);
-        for (int32_t slot = 0; slot < n_own + n_halo; ++slot) {
-            const int32_t he = slot < n_own ? h0 + slot : L.halo_he[L.halo_off[b] + slot - n_own];
-            const int32_t h = L.he_old[he];
-            const int32_t owner = L.iperm[g.he_face[h]];
-            const int32_t val = g.foff[g.he_face[h] + 1] - g.foff[g.he_face[h]];
-            int32_t* row = &L.ext_loc[4 * size_t(h0 + L.halo_off[b] + slot)];
-            row[0] = loc(owner) | (val << 16);
-            row[1] = loc(L.iperm[g.ff_plus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_plus[2 * size_t(h) + 1]]) << 16);
-            row[2] = loc(L.iperm[g.ff_minus[2 * size_t(h)]]) | (loc(L.iperm[g.ff_minus[2 * size_t(h) + 1]]) << 16);
-            row[3] = he;
+            if (int32_t(fs.size()) > Layout::kFaceSetMax) ++too_big;
+            for (int32_t f : fs) local[f] = -1;
         }
-        if (int32_t(L.fset_face.size() - first) > Layout::kFaceSetMax)
-            throw std::runtime_error("a block's stencil face set exceeds the LDS face slots");
-        for (size_t q = first; q < L.fset_face.size(); ++q) local[L.fset_face[q]] = -1;
-        L.fset_off.push_back(int32_t(L.fset_face.size()));
     }
+    if (too_big) throw std::runtime_error("a block's stencil face set exceeds the LDS face slots");
+    L.fset_off.assign(size_t(nb) + 1, 0);
+    for (int32_t b = 0; b < nb; ++b) L.fset_off[size_t(b) + 1] = L.fset_off[size_t(b)] + int32_t(sets[size_t(b)].size());
+    L.fset_face.assign(size_t(L.fset_off[size_t(nb)]), 0);
+#pragma omp parallel for schedule(static)
+    for (int32_t b = 0; b < nb; ++b)
+        std::copy(sets[size_t(b)].begin(), sets[size_t(b)].end(), L.fset_face.begin() + L.fset_off[size_t(b)]);
     return L;
 }
 
