@@ -55,12 +55,14 @@ def test_host_mesh_and_geometry_are_bit_identical_to_the_oracle():
         om, pm = O.Mesh(kind, *args), gen(*args)
         assert np.array_equal(om.xyz.view(np.uint64), pm.vertices.view(np.uint64))
         assert np.array_equal(om.foff, pm.face_offsets) and np.array_equal(om.fv, pm.face_vertices)
-        og, pg = O.Geometry(om), S.Geometry(pm, threads=4)
-        for f in ("fc", "normal", "area", "nb_edge", "twin", "ff_plus", "ff_minus", "beta_plus", "beta_minus",
-                  "h_plus", "h_minus", "bm_dist", "en"):
-            a, b = np.asarray(getattr(og, f)), np.asarray(getattr(pg, f))
-            assert a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8)), (kind, f)
-        assert og.h0 == pg.h0 and og.round_diff == pg.round_diff
+        og = O.Geometry(om)
+        for threads in (4, 0):  # 0: every host thread; the tables must not depend on the count
+            pg = S.Geometry(pm, threads=threads)
+            for f in ("fc", "normal", "area", "nb_edge", "twin", "ff_plus", "ff_minus", "beta_plus", "beta_minus",
+                      "h_plus", "h_minus", "bm_dist", "en"):
+                a, b = np.asarray(getattr(og, f)), np.asarray(getattr(pg, f))
+                assert a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8)), (kind, f, threads)
+            assert og.h0 == pg.h0 and og.round_diff == pg.round_diff
 
 
 def test_argument_errors_are_codes_not_crashes():
