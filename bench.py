@@ -278,7 +278,7 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's step count)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dt", type=float, default=1e-4)
-    ap.add_argument("--cpu-face-updates", type=float, default=3e7,
+    ap.add_argument("--cpu-face-updates", type=float, default=5e7,
                     help="size of the CPU baseline sample in face-updates (whole steps, at least one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble (C4) measurement")
