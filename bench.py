@@ -6,10 +6,13 @@ Default workload (C3, the mixed-precision configuration): quad_sphere(9)
 omega_acc 0.8, tilt 0.3 rad, width 0.1), friction on with alpha = 1e-6 (the
 shipped parameter file's values), dt0 = 1e-4 (the CFL clamp may lower it:
 the effective dt is reported), 1000 steps; seeded state Sigma = 1 +
-0.1 N(0,1), Pi = 0.1 Sigma, solid-body rotation Omega = 0.5 about z plus
-0.01 N(0,1) tangential noise.  `--workload` selects the other single-mesh
-configurations (c2: icosphere_hex(7) HLLC+; c5a / c5b: uv_sphere(2048,1024)
-/ icosphere_hex(9) HLLC+ with a spherical-harmonic density).
+0.1 N(0,1), Pi = 0.1 Sigma, solid-body rotation Omega = 0.3 about z plus
+0.01 N(0,1) tangential noise (at Omega = 0.3 the effective gravity stays
+positive, so the friction dt limit is live and stop_check stays clear; at
+Omega = 0.5, `--state rotation05`, the reference driver would stop on step
+1).  `--workload` selects the other single-mesh configurations (c2:
+icosphere_hex(7) HLLC+ at Omega = 0.5; c5a / c5b: uv_sphere(2048,1024) /
+icosphere_hex(9) HLLC+ with a spherical-harmonic density).
 
 Both precision paths run the same steps from the same state:
   value       = face-updates/s of the FP32-flux / FP64-state path,
@@ -26,6 +29,7 @@ the oracle (the C restatement of the reference step) on a bounded sample of
 the same workload.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -45,18 +49,32 @@ ACCRETION = dict(accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1, omega_
 
 # name: (mesh kind, mesh args, solver, omega_ns, accretion, initial state, steps)
 WORKLOADS = {
-    "c3": ("quad", (9,), "HLLC", 0.1, True, "rotation", 1000),
-    "c2": ("hex", (7,), "HLLCplus", 0.0, False, "rotation", 1000),
+    "c3": ("quad", (9,), "HLLC", 0.1, True, "rotation03", 1000),
+    "c2": ("hex", (7,), "HLLCplus", 0.0, False, "rotation05", 1000),
     "c5a": ("uv", (2048, 1024), "HLLCplus", 0.1, True, "harmonics", 2000),
     "c5b": ("hex", (9,), "HLLCplus", 0.1, True, "harmonics", 2000),
 }
 
-# HBM bytes per launch of the FP32-flux stage kernels on the default workload,
-# from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script,
-# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (a 256 MiB device copy reads
-# 128 MiB of FETCH_SIZE and 256 MiB of WRITE_SIZE).  Valid for the kernels of
-# this source tree only; the summary with the commands is under profiles/.
-MEASURED_TRAFFIC = {("c3", "stage_1"): 535_012_576, ("c3", "stage_2"): 598_613_458}
+# HBM bytes per launch of the FP32-flux stage kernels, from rocprofv3 --pmc
+# FETCH_SIZE and --pmc WRITE_SIZE runs of this script, FETCH_SIZE scaled by
+# 2.0 and WRITE_SIZE by 1.0 (a 256 MiB device copy reads 128 MiB of
+# FETCH_SIZE and 256 MiB of WRITE_SIZE).  Each entry holds for the kernel
+# sources whose hash it names (kernel_source_hash); for any other source tree
+# the bench line reports traffic as null.  The summary with the commands is
+# under profiles/.
+MEASURED_TRAFFIC = {}
+KERNEL_SOURCES = ["stage_mixed.hip", "stage_kernels.inc", "engine_types.hpp", "launch.hpp", "layout.cpp",
+                  "run_setup.cpp"]
+
+
+def kernel_source_hash():
+    """First 16 hex digits of the SHA-256 of the sources that decide what the
+    mixed stage kernels read and write."""
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "muscl-scheme-on-spherical-mesh-wip_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def algorithmic_bytes(kernel, F, H, d=5):
@@ -80,19 +98,25 @@ def make_mesh(S, kind, args):
 
 def initial_state(S, mesh, ic):
     fc = mesh.face_centres()
-    if ic == "rotation":
-        return S.ic.adiabatic_state(fc, GAM, seed=2, sigma_amp=0.1, pi_frac=0.1, omega=(0.0, 0.0, 0.5), noise=0.01)
+    if ic.startswith("rotation"):
+        om = {"rotation03": 0.3, "rotation05": 0.5}[ic]
+        return S.ic.adiabatic_state(fc, GAM, seed=2, sigma_amp=0.1, pi_frac=0.1, omega=(0.0, 0.0, om), noise=0.01)
     sigma = 1.0 + S.ic.real_sph_harm_sum(fc, seed=5, n_modes=8, lmax=16, amplitude=0.05)
     return S.ic.adiabatic_state(fc, GAM, seed=5, sigma=sigma, pressure=np.full_like(sigma, 0.1), noise=0.0)
 
 
-def params(S, accretion):
-    return S.default_params(setup_threads=16, **(ACCRETION if accretion else {}))
+NO_GRAPH = False  # --no-graph: every step launched on its own (kernel traces)
+
+
+def params(S, accretion, **extra):
+    return S.default_params(setup_threads=16, no_graph=int(NO_GRAPH), **(ACCRETION if accretion else {}), **extra)
 
 
 def run_path(S, mesh, U0, wl, precision, steps, warmup, dt0):
     kind, margs, solver, omega_ns, acc, ic, _ = wl
     s = getattr(S, "MUSCL_" + solver)(mesh, U0, 5, GAM, omega_ns, acc, params=params(S, acc), precision=precision)
+    # the first do_steps call captures the step graph after its own steps,
+    # so the warm-up (at least one step) keeps the capture out of the timing
     s.do_steps(dt0, warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -164,8 +188,7 @@ def ensemble_bench(S, steps, warmup, dt0, profile_steps):
     runs = ensemble_runs(S)
     p = params(S, False)
     t_setup = time.perf_counter()
-    ens = S.Ensemble([(m, U0, GAM, 0.0, S.default_params(setup_threads=16, patch_faces=-1)) for m, U0, _ in runs],
-                     5, S.HLLCPLUS)
+    ens = S.Ensemble([(m, U0, GAM, 0.0, params(S, False, patch_faces=-1)) for m, U0, _ in runs], 5, S.HLLCPLUS)
     t_setup = time.perf_counter() - t_setup
     n_runs, F, H, blocks = ens.info()
     tuned_faces = ens.patch_faces()
@@ -275,8 +298,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+    ap.add_argument("--state", choices=["rotation03", "rotation05", "harmonics"], default=None,
+                    help="initial state (default: the workload's own)")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's step count)")
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20, help="untimed steps before the timed ones (at least 1)")
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--cpu-face-updates", type=float, default=5e7,
                     help="size of the CPU baseline sample in face-updates (whole steps, at least one)")
@@ -286,9 +311,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--driver-steps", type=int, default=200,
                     help="iterations of the reference driver's per-step loop (do_step + readouts) to time")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step on its own instead of replaying captured HIP graphs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    args.warmup = max(1, args.warmup)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
@@ -304,8 +332,12 @@ def main():
         print(f"copy calibration: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written",
               file=sys.stderr)
 
+    global NO_GRAPH
+    NO_GRAPH = args.no_graph
     import splash_amd as S
     wl = WORKLOADS[args.workload]
+    if args.state:
+        wl = wl[:5] + (args.state,) + wl[6:]
     kind, margs, solver, omega_ns, acc, ic, wl_steps = wl
     steps = args.steps or wl_steps
     mesh = make_mesh(S, kind, margs)
@@ -337,11 +369,13 @@ def main():
     achieved = ab / (prof[k] * 1e-3) / 1e9
     copy_gbs = copy_bandwidth()
     step_bytes = 2 * (104 * H + 36 * F) + 40 * 5 * F
-    traffic = MEASURED_TRAFFIC.get((args.workload, name))
+    src_hash = kernel_source_hash()
+    traffic = MEASURED_TRAFFIC.get((args.workload, ic, name, src_hash))
 
     mesh_name = {"hex": "icosphere_hex", "quad": "quad_sphere", "uv": "uv_sphere"}[kind]
-    state = ("Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.5 z + 0.01N" if ic == "rotation"
-             else "Sigma=1+8 real Y_lm (l<=16, amplitude 0.05), Pi=0.1, at rest")
+    state = ({"rotation03": "Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.3 z + 0.01N",
+              "rotation05": "Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.5 z + 0.01N"}.get(ic)
+             or "Sigma=1+8 real Y_lm (l<=16, amplitude 0.05), Pi=0.1, at rest")
     sources = ("accretion on (rate 1e-3, e_acc 0.1, omega_acc 0.8, tilt 0.3, width 0.1), friction alpha 1e-6"
                if acc else "accretion off")
     line = {
@@ -373,6 +407,7 @@ def main():
         "step_achieved_GBs": step_bytes / (prof[3] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel_source_hash": src_hash,
                      "algorithmic_bytes": ab, "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "stop_check": bool(stop),
         "sim_time": [t64, tmx],

@@ -80,6 +80,8 @@ typedef struct splash_params {
                             * (OpenMP's default team); the tables do not depend on it */
     int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..64; 0 = 64;
                           * -1 = the fastest of a few candidates, measured at creation */
+    int32_t no_graph;    /* 0: splash_do_steps replays a captured HIP graph (captured after the
+                          * first call's steps); 1: every step is launched on its own */
 } splash_params;
 
 typedef struct splash_mesh splash_mesh;   /* library-owned generated mesh */
@@ -113,7 +115,9 @@ void splash_destroy(splash_ctx *ctx);
 /* MUSCL_base::do_step(dt0) (MUSCL_base.hpp:97-225).  Enqueues the step on
  * the context's stream and returns; dt never leaves the GPU. */
 int splash_do_step(splash_ctx *ctx, double dt0);
-/* n consecutive do_step(dt0) calls, replayed from a captured HIP graph. */
+/* n consecutive do_step(dt0) calls.  The first call launches its steps one
+ * by one and then captures 8 steps into a HIP graph (unless
+ * params.no_graph); later calls replay it for each group of 8 steps. */
 int splash_do_steps(splash_ctx *ctx, double dt0, int n);
 /* Measurement extension: runs n do_step(dt0) calls back to back on the
  * context's stream, each kernel launched with start/stop HIP events bound
@@ -205,7 +209,9 @@ int splash_ensemble_create(int n_runs, const splash_mesh_desc *meshes, const dou
                            splash_ensemble **out);
 void splash_ensemble_destroy(splash_ensemble *e);
 /* nsteps do_step calls of every run, dt0[r] being run r's requested step;
- * replayed from a captured HIP graph, returns without waiting. */
+ * returns without waiting.  As splash_do_steps, the first call captures a
+ * HIP graph of 8 steps after launching its own (unless params[0].no_graph)
+ * and later calls replay it. */
 int splash_ensemble_step(splash_ensemble *e, const double *dt0, int nsteps);
 /* Measurement extension, as splash_profile_step: n steps of every run with
  * dispatch-bound HIP events; ms_out[4] = mean [dt finalize, stage 1,

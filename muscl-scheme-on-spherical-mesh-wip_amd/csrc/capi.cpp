@@ -115,6 +115,7 @@ void splash_default_params(splash_params* p) {
     p->precision = SPLASH_FP64;
     p->setup_threads = 0;
     p->patch_faces = 0;
+    p->no_graph = 0;
 }
 
 int splash_mesh_uv_sphere(int n_slices, int n_stacks, splash_mesh** out) {
@@ -155,6 +156,7 @@ int splash_create(const splash_mesh_desc* mesh, const double* U_in, int dim, int
         auto ctx = std::make_unique<splash_ctx>();
         ctx->engine = std::make_unique<splash::Engine>(m, U_in, dim, solver, gam, omega_ns, rp, p.precision,
                                                        p.setup_threads, p.patch_faces);
+        ctx->engine->set_use_graph(!p.no_graph);
         *out = ctx.release();
         return SPLASH_OK;
     });
@@ -378,6 +380,7 @@ int splash_ensemble_create(int n_runs, const splash_mesh_desc* meshes, const dou
         auto e = std::make_unique<splash_ensemble>();
         e->ens = std::make_unique<splash::Ensemble>(runs, dim, solver, params[0].setup_threads,
                                                     params[0].patch_faces);
+        e->ens->set_use_graph(!params[0].no_graph);
         *out = e.release();
         return SPLASH_OK;
     });

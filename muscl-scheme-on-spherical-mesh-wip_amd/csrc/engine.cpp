@@ -61,6 +61,11 @@ void Engine::init(const RunHost& r, const double* U_in) {
         enqueue_prepare();
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
         for (void* p : allocations_) (void)hipFree(p);
         allocations_.clear();
         if (h_readout_) (void)hipHostFree(h_readout_);
@@ -93,9 +98,11 @@ int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0) {
 }
 
 Engine::~Engine() {
+    // graph launches still in flight use the executable graph and every
+    // buffer: wait for them before anything is released
+    if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
     if (h_readout_) (void)hipHostFree(h_readout_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -171,7 +178,11 @@ void Engine::upload(const RunHost& r) {
 
     args_.phys = r.phys;
     args_.phys_dev = static_cast<const PhysParams*>(dev(sizeof(PhysParams), &r.phys));
-    args_.dt0_dev = nullptr;
+    args_.dt0_dev = nullptr;  // eager steps take dt0 as a kernel argument
+    {
+        const double zero = 0;
+        dt0_dev_ = static_cast<double*>(dev(sizeof(double), &zero));
+    }
     args_.U = static_cast<double*>(dev(sizeof(double) * size_t(nf) * dim_, nullptr));
     args_.W = args_.W2 = nullptr;  // FP64 p-form stage inputs (the mixed path keeps FP32 ones)
     if (precision_ == kFP64) {
@@ -207,10 +218,11 @@ void Engine::enqueue_prepare() {
     check(hipGetLastError(), "prepare launch");
 }
 
-void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev) {
-    if (which == 0) fp64::enqueue_finalize_dt(args_, dt0, stream_, ev);
-    else if (precision_ == kFP64) fp64::enqueue_stage(args_, which, stream_, ev);
-    else mixed::enqueue_stage(args_, which, stream_, ev);
+void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev, const StepArgs* a) {
+    const StepArgs& args = a ? *a : args_;
+    if (which == 0) fp64::enqueue_finalize_dt(args, dt0, stream_, ev);
+    else if (precision_ == kFP64) fp64::enqueue_stage(args, which, stream_, ev);
+    else mixed::enqueue_stage(args, which, stream_, ev);
 }
 
 void Engine::enqueue_step(double dt0) {
@@ -258,23 +270,45 @@ void Engine::step(double dt0) {
     check(hipGetLastError(), "step launch");
 }
 
+void Engine::capture_graph() {
+    // the graph's dt kernel reads dt0 from dt0_dev_, so this one graph serves
+    // every dt0; a failed capture is ended before the error propagates
+    StepArgs ga = args_;
+    ga.dt0_dev = dt0_dev_;
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        for (int k = 0; k < kGraphSteps; ++k)
+            for (int w = 0; w < kKernelsPerStep; ++w) enqueue_kernel(w, 0.0, nullptr, &ga);
+        check(hipGetLastError(), "capture launch");
+    } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(stream_, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+    }
+    check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
 void Engine::steps(double dt0, int n) {
     if (n <= 0) return;
     stepped_ = true;
     readout_valid_ = false;
-    constexpr int kGraphSteps = 8;
+    if (!graph_exec_) {
+        // the first call launches its steps one by one and captures the
+        // graph afterwards (nothing is launched while capturing), so every
+        // kernel has run before it is captured and later calls replay it
+        for (int k = 0; k < n; ++k) enqueue_step(dt0);
+        check(hipGetLastError(), "steps launch");
+        if (use_graph_) capture_graph();
+        return;
+    }
     if (n >= kGraphSteps) {
-        if (!graph_exec_ || graph_dt0_ != dt0 || graph_steps_ != kGraphSteps) {
-            if (graph_exec_) check(hipGraphExecDestroy(graph_exec_), "hipGraphExecDestroy");
-            if (graph_) check(hipGraphDestroy(graph_), "hipGraphDestroy");
-            graph_exec_ = nullptr;
-            graph_ = nullptr;
-            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (int k = 0; k < kGraphSteps; ++k) enqueue_step(dt0);
-            check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
-            check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (dt0 != graph_dt0_ || !graph_dt0_set_) {
+            // stream-ordered: replays already queued keep reading the old value
+            fp64::enqueue_set_scalar(dt0_dev_, dt0, stream_);
             graph_dt0_ = dt0;
-            graph_steps_ = kGraphSteps;
+            graph_dt0_set_ = true;
         }
         while (n >= kGraphSteps) {
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");

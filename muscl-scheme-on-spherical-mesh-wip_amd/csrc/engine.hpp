@@ -37,7 +37,13 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     void step(double dt0);              // enqueue one do_step
-    void steps(double dt0, int n);      // enqueue n do_steps from a captured graph
+    // enqueue n do_steps: the first call launches them one by one and then
+    // captures kGraphSteps steps into a HIP graph; later calls replay that
+    // graph for each group of kGraphSteps and launch the rest one by one
+    void steps(double dt0, int n);
+    static constexpr int kGraphSteps = 8;
+    // false: steps() never captures a graph (every step is launched eagerly)
+    void set_use_graph(bool on) { use_graph_ = on; }
     void synchronize();
     // Runs n eager steps with HIP events around each launch; ms receives the
     // average duration of [finalize_dt, stage 1, stage 2, whole step].
@@ -74,7 +80,10 @@ public:
 private:
     void enqueue_step(double dt0);
     void enqueue_prepare();
-    void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr);
+    // a: the launch arguments (default: args_, whose dt kernel takes dt0 by
+    // value)
+    void enqueue_kernel(int which, double dt0, const KernelEvents* ev = nullptr, const StepArgs* a = nullptr);
+    void capture_graph();
     void init(const RunHost& r, const double* U_in);
     void upload(const RunHost& r);
     Scalars read_scalars();
@@ -113,10 +122,14 @@ private:
     bool stepped_ = false;              // a step ran since the state was set
     double* d_curl_ = nullptr;          // lazily: ev [nh][3], out [nf], W_tmp [nf][dim]
 
+    // kGraphSteps steps captured after the first steps() call; their dt
+    // kernel reads dt0 from dt0_dev_, last set to graph_dt0_
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
+    double* dt0_dev_ = nullptr;
     double graph_dt0_ = 0;
-    int graph_steps_ = 0;
+    bool graph_dt0_set_ = false;
+    bool use_graph_ = true;
 };
 
 // Mixed-precision patch face cap with the shortest measured step: each

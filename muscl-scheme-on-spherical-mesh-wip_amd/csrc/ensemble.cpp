@@ -221,6 +221,11 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         check(hipGetLastError(), "prepare launch");
         check(hipStreamSynchronize(stream_), "prepare");
     } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
         for (void* p : allocations_) (void)hipFree(p);
         if (h_sums_) (void)hipHostFree(h_sums_);
         if (h_scalars_) (void)hipHostFree(h_scalars_);
@@ -230,9 +235,11 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
 }
 
 Ensemble::~Ensemble() {
+    // graph launches still in flight use the executable graph and every
+    // buffer: wait for them before anything is released
+    if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocations_) (void)hipFree(p);
     if (h_sums_) (void)hipHostFree(h_sums_);
     if (h_scalars_) (void)hipHostFree(h_scalars_);
@@ -258,22 +265,38 @@ void Ensemble::enqueue_step() {
     for (int k = 0; k < kKernelsPerStep; ++k) enqueue_kernel(k);
 }
 
+void Ensemble::capture_graph() {
+    // dt0 lives in device memory, so this one graph serves every call; a
+    // failed capture is ended before the error propagates
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        for (int k = 0; k < kGraphSteps; ++k) enqueue_step();
+        check(hipGetLastError(), "capture launch");
+    } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(stream_, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+    }
+    check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
 void Ensemble::steps(const double* dt0, int n) {
     upload_dt0(dt0);
     if (n <= 0) return;
     readout_valid_ = false;
-    constexpr int kGraphSteps = 8;
-    if (n >= kGraphSteps) {
-        if (!graph_exec_) {  // dt0 lives in device memory, so one graph serves every call
-            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (int k = 0; k < kGraphSteps; ++k) enqueue_step();
-            check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
-            check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
-        }
-        while (n >= kGraphSteps) {
-            check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
-            n -= kGraphSteps;
-        }
+    if (!graph_exec_) {
+        // the first call launches its steps one by one and captures the
+        // graph afterwards, so every kernel has run before it is captured
+        for (int k = 0; k < n; ++k) enqueue_step();
+        check(hipGetLastError(), "steps launch");
+        if (use_graph_) capture_graph();
+        return;
+    }
+    while (n >= kGraphSteps) {
+        check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        n -= kGraphSteps;
     }
     for (int k = 0; k < n; ++k) enqueue_step();
     check(hipGetLastError(), "steps launch");

@@ -31,7 +31,13 @@ public:
     Ensemble(const Ensemble&) = delete;
     Ensemble& operator=(const Ensemble&) = delete;
 
-    void steps(const double* dt0, int n);  // dt0: [n_runs]; n steps, graph-replayed in groups of 8
+    // dt0: [n_runs]; n steps: the first call launches them one by one and
+    // then captures kGraphSteps steps into a HIP graph; later calls replay it
+    // for each group of kGraphSteps and launch the rest one by one
+    void steps(const double* dt0, int n);
+    static constexpr int kGraphSteps = 8;
+    // false: steps() never captures a graph
+    void set_use_graph(bool on) { use_graph_ = on; }
     void synchronize();
     static constexpr int kKernelsPerStep = 3;
     // n back-to-back steps, each kernel timed with dispatch-bound HIP events:
@@ -62,6 +68,7 @@ private:
     void init(const std::vector<RunHost>& hosts, const std::vector<const double*>& U_in);
     void upload_dt0(const double* dt0);
     void enqueue_step();
+    void capture_graph();
     void enqueue_kernel(int which, const KernelEvents* ev = nullptr);
     Scalars read_scalars(int r);
     void ensure_readout();
@@ -96,6 +103,7 @@ private:
     bool readout_valid_ = false;
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
+    bool use_graph_ = true;
 };
 
 }  // namespace splash

@@ -55,6 +55,8 @@ void enqueue_readout_runs(const StepArgs& a, const uint8_t* light_mask, const do
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s);
 int block_size();
 int finalize_blocks();  // dt-kernel workgroups per run
+// *p = v, ordered on stream s (a one-thread kernel: no memset node in a graph)
+void enqueue_set_scalar(double* p, double v, hipStream_t s);
 }  // namespace fp64
 
 namespace mixed {

@@ -85,6 +85,12 @@ void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_t
     else k_curl<4><<<gf, kBlock, 0, s>>>(a.geom, a.phys, a.sc, a.U, W, ev, out);
 }
 
+__global__ void k_set_scalar(double* p, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+void enqueue_set_scalar(double* p, double v, hipStream_t s) { k_set_scalar<<<1, 64, 0, s>>>(p, v); }
+
 int block_size() { return kBlock; }
 int finalize_blocks() { return kFinalizeBlocks; }
 

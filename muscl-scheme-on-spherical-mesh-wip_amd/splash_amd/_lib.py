@@ -32,7 +32,8 @@ class Params(C.Structure):
     _fields_ = [("accretion_on", C.c_int32), ("friction_on", C.c_int32),
                 ("acc_rate", C.c_double), ("e_acc", C.c_double), ("omega_acc_abs", C.c_double),
                 ("tilt_angle", C.c_double), ("acc_width", C.c_double), ("alpha", C.c_double),
-                ("precision", C.c_int32), ("setup_threads", C.c_int32), ("patch_faces", C.c_int32)]
+                ("precision", C.c_int32), ("setup_threads", C.c_int32), ("patch_faces", C.c_int32),
+                ("no_graph", C.c_int32)]
 
 
 class GeometryView(C.Structure):
