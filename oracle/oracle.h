@@ -92,6 +92,12 @@ double or_area_coeff(const or_solver *s);                      /* adiabatic.hpp:
 void or_destroy(or_solver *s);
 
 /* Single-function entry points for closed-form tests. */
+/* source (adiabatic.hpp:396-541) of the p-form state u at unit face centre
+ * fcn, with the solver's parameters, area_coeff and total_mass. */
+void or_source(const or_solver *s, const double fcn[3], const double u[5], double res[5]);
+/* find_U_edges (MUSCL_base.hpp:361-434) of the current state with step dt:
+ * Uplus_out [nh][dim] (slot 4 = E_fc for dim 5); the state is unchanged. */
+void or_reconstruct(or_solver *s, double dt, double *Uplus_out);
 void or_limiter_adiabatic(const double r[5], double nu, double eta_minus, double eta_plus,
                           int valence, double lim_out[5]);     /* adiabatic.hpp:685-828 */
 /* Riemann flux of one half-edge given the geometry of that half-edge:

@@ -24,11 +24,17 @@
  *   - the area_coeff loop (adiabatic.hpp:75, uninitialised start index)
  *     starts at face 0.
  *
- * Parity unpinned (no reference build exists in this image to produce
- * golden vectors -- pmp's mat_vec.h:17 needs the absent Eigen headers):
- * the composition of the limiter blend with the dt-dependent Courant
- * number, and the glibc libm values of pow/tanh/asin/acos/exp; these are
- * covered by closed-form values and invariants only, see tests/.
+ * Pins (tests/test_oracle_*.py): closed-form flux, wave-speed and limiter
+ * values; conservation and rest-state invariants of do_step; for source(),
+ * the accretion profile's half-maximum width and centre, the t = 0 mass
+ * balance, the continuum limit of area_coeff, the accreted angular momentum
+ * and energy, the cooling law on both branches, its continuity at the
+ * switch and its clamps, and the friction drag law; for find_U_edges,
+ * second-order convergence to smooth fields and the Courant-number
+ * composition of the limiter at a large step.  Parity unpinned: only the
+ * last bits of glibc libm's pow/tanh/asin/acos/exp/atan2 (no reference
+ * build exists in this image to produce golden vectors -- pmp's mat_vec.h:17
+ * needs the absent Eigen headers).
  */
 #include "oracle.h"
 #include "oracle_vec.h"
@@ -358,15 +364,14 @@ static void limiter(const or_solver *s, const double *W, int f, int j, int h, co
 
 /* ---------------------------------------------------------- sources */
 
-/* adiabatic.hpp:396-541 (u[4] is pressure) */
-static void source(const or_solver *s, const double *u, int f, double *res)
+/* adiabatic.hpp:396-541 (u[4] is pressure) at the unit face centre fcn */
+static void source_at(const or_solver *s, const double *u, v3 fcn, double *res)
 {
     const int dim = s->dim;
     for (int i = 0; i < dim; i++) res[i] = 0;
     if (!is_adiabatic(s)) return; /* isothermal.hpp:199-208 */
     const double gam = s->gam;
     v3 l = v3m(u[1], u[2], u[3]);
-    v3 fcn = face_center_unit(s, f);
     v3 vel = v3divs(v3cross(fcn, l), -u[0]);
     double mu = M_PI / 2;
     double sigma = s->p.acc_width / 2.355;
@@ -425,6 +430,16 @@ static void source(const or_solver *s, const double *u, int f, double *res)
             res[4] -= s->p.alpha * rho * dn * (vn * vn / 2. + u[4] * gam / ((gam - 1) * u[0]));
         }
     }
+}
+
+static void source(const or_solver *s, const double *u, int f, double *res)
+{
+    source_at(s, u, face_center_unit(s, f), res);
+}
+
+void or_source(const or_solver *s, const double fcn[3], const double u[5], double res[5])
+{
+    source_at(s, u, v3ld(fcn), res);
 }
 
 /* adiabatic.hpp:543-580 (U[i][4] is energy here); isothermal: 1e20 */
@@ -717,6 +732,18 @@ void or_curl(const or_solver *s, double *out)
         }
         out[f] = vort / s->g->area[f];
     }
+}
+
+void or_reconstruct(or_solver *s, double dt, double *Uplus_out)
+{
+    const size_t n = (size_t)s->nf * s->dim;
+    double dt_saved = s->dt;
+    memcpy(s->U_temp, s->U, sizeof(double) * n);
+    s->dt = dt;
+    find_U_edges(s);
+    memcpy(Uplus_out, s->Uplus, sizeof(double) * (size_t)s->nh * s->dim);
+    memcpy(s->U, s->U_temp, sizeof(double) * n);
+    s->dt = dt_saved;
 }
 
 void or_get_state(const or_solver *s, double *U_out)

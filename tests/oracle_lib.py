@@ -75,6 +75,8 @@ def lib():
         L.or_diagnostics.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.or_curl.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.or_destroy.argtypes = [C.c_void_p]
+        L.or_source.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_reconstruct.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double)]
         L.or_limiter_adiabatic.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double,
                                            C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.or_flux_star.restype = C.c_int
@@ -194,6 +196,20 @@ class Solver:
     def curl(self):
         out = np.empty(self.mesh.nf)
         lib().or_curl(self.ptr, _dp(out))
+        return out
+
+    def source(self, fcn, u):
+        """source() of p-form state u at unit face centre fcn."""
+        fcn = np.ascontiguousarray(fcn, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(5)
+        lib().or_source(self.ptr, _dp(fcn), _dp(u), _dp(out))
+        return out
+
+    def reconstruct(self, dt):
+        """U_plus [nh][dim] of find_U_edges on the current state with step dt."""
+        out = np.empty((self.mesh.nh, self.dim))
+        lib().or_reconstruct(self.ptr, dt, _dp(out))
         return out
 
     def __del__(self):
