@@ -42,18 +42,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-GAM = 2 - 1 / (4 / 3)
 KERNELS = ["finalize_dt", "stage_1", "stage_2"]
-ACCRETION = dict(accretion_on=1, friction_on=1, acc_rate=1e-3, e_acc=0.1, omega_acc_abs=0.8, tilt_angle=0.3,
-                 acc_width=0.1, alpha=1e-6)
 
-# name: (mesh kind, mesh args, solver, omega_ns, accretion, initial state, steps)
-WORKLOADS = {
-    "c3": ("quad", (9,), "HLLC", 0.1, True, "rotation03", 1000),
-    "c2": ("hex", (7,), "HLLCplus", 0.0, False, "rotation05", 1000),
-    "c5a": ("uv", (2048, 1024), "HLLCplus", 0.1, True, "harmonics", 2000),
-    "c5b": ("hex", (9,), "HLLCplus", 0.1, True, "harmonics", 2000),
-}
+from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state, make_mesh  # noqa: E402
 
 # HBM bytes per launch of the FP32-flux stage kernels, from rocprofv3 --pmc
 # FETCH_SIZE and --pmc WRITE_SIZE runs of this script, FETCH_SIZE scaled by
@@ -92,19 +83,6 @@ def algorithmic_bytes(kernel, F, H, d=5):
     return 0
 
 
-def make_mesh(S, kind, args):
-    return {"hex": S.icosphere_hex, "quad": S.quad_sphere, "uv": S.uv_sphere}[kind](*args)
-
-
-def initial_state(S, mesh, ic):
-    fc = mesh.face_centres()
-    if ic.startswith("rotation"):
-        om = {"rotation03": 0.3, "rotation05": 0.5}[ic]
-        return S.ic.adiabatic_state(fc, GAM, seed=2, sigma_amp=0.1, pi_frac=0.1, omega=(0.0, 0.0, om), noise=0.01)
-    sigma = 1.0 + S.ic.real_sph_harm_sum(fc, seed=5, n_modes=8, lmax=16, amplitude=0.05)
-    return S.ic.adiabatic_state(fc, GAM, seed=5, sigma=sigma, pressure=np.full_like(sigma, 0.1), noise=0.0)
-
-
 NO_GRAPH = False  # --no-graph: every step launched on its own (kernel traces)
 
 
@@ -113,7 +91,7 @@ def params(S, accretion, **extra):
 
 
 def run_path(S, mesh, U0, wl, precision, steps, warmup, dt0):
-    kind, margs, solver, omega_ns, acc, ic, _ = wl
+    kind, margs, solver, omega_ns, acc, ic, _, _ = wl
     s = getattr(S, "MUSCL_" + solver)(mesh, U0, 5, GAM, omega_ns, acc, params=params(S, acc), precision=precision)
     # the first do_steps call captures the step graph after its own steps,
     # so the warm-up (at least one step) keeps the capture out of the timing
@@ -160,32 +138,12 @@ def driver_loop(s, dt0, n):
     return out
 
 
-def ensemble_runs(S, n_runs=64, seed=4):
-    """C4: n_runs icosphere_hex meshes with levels drawn from {4, 5, 6, 7}
-    (seed 4), run i seeded 4000 + i: Sigma = 1 + 0.1 N, Pi = 0.1 Sigma,
-    solid-body rotation about a uniform random axis with Omega ~ U(0, 1)
-    plus 0.01 N tangential noise."""
-    levels = 4 + np.minimum((S.ic.uniform(seed, 0, n_runs) * 4).astype(int), 3)
-    meshes = {lv: S.icosphere_hex(int(lv)) for lv in sorted(set(levels.tolist()))}
-    runs = []
-    for i, lv in enumerate(levels.tolist()):
-        si = 4000 + i
-        axis = S.ic.gaussian(si, 7, 3)
-        axis /= np.linalg.norm(axis)
-        omega = S.ic.uniform(si, 8, 1)[0] * axis
-        m = meshes[lv]
-        U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=si, sigma_amp=0.1, pi_frac=0.1,
-                                  omega=tuple(omega), noise=0.01)
-        runs.append((m, U0, lv))
-    return runs
-
-
 def ensemble_bench(S, steps, warmup, dt0, profile_steps):
     """The C4 ensemble: one launch per stage for all 64 runs, its patch shape
     chosen by the library's tuner (patch_faces = -1), against the same runs
     advanced one after another by single solvers (FP32-flux and FP64, default
     patch shape; results do not depend on it)."""
-    runs = ensemble_runs(S)
+    runs = c4_runs(S)
     p = params(S, False)
     t_setup = time.perf_counter()
     ens = S.Ensemble([(m, U0, GAM, 0.0, params(S, False, patch_faces=-1)) for m, U0, _ in runs], 5, S.HLLCPLUS)
@@ -242,7 +200,7 @@ def ensemble_bench(S, steps, warmup, dt0, profile_steps):
 
 def cpu_baseline(S, wl, U0, dt0, steps, threads):
     import oracle_lib as O
-    kind, margs, solver, omega_ns, acc, ic, _ = wl
+    kind, margs, solver, omega_ns, acc, ic, _, _ = wl
     m = O.Mesh(kind, *margs)
     p = O.make_params(**{k: v for k, v in ACCRETION.items()}) if acc else O.make_params()
     osolver = {"HLLCplus": O.HLLCPLUS, "HLLC": O.HLLC}[solver]
@@ -297,7 +255,7 @@ def copy_bandwidth(mib=1024, reps=10):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+    ap.add_argument("--workload", choices=sorted(k for k in WORKLOADS if k != "c1"), default="c3")
     ap.add_argument("--state", choices=["rotation03", "rotation05", "harmonics"], default=None,
                     help="initial state (default: the workload's own)")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's step count)")
@@ -338,7 +296,7 @@ def main():
     wl = WORKLOADS[args.workload]
     if args.state:
         wl = wl[:5] + (args.state,) + wl[6:]
-    kind, margs, solver, omega_ns, acc, ic, wl_steps = wl
+    kind, margs, solver, omega_ns, acc, ic, wl_steps, _ = wl
     steps = args.steps or wl_steps
     mesh = make_mesh(S, kind, margs)
     F, H = mesh.n_faces, mesh.n_halfedges
@@ -373,9 +331,7 @@ def main():
     traffic = MEASURED_TRAFFIC.get((args.workload, ic, name, src_hash))
 
     mesh_name = {"hex": "icosphere_hex", "quad": "quad_sphere", "uv": "uv_sphere"}[kind]
-    state = ({"rotation03": "Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.3 z + 0.01N",
-              "rotation05": "Sigma=1+0.1N, Pi=0.1Sigma, Omega=0.5 z + 0.01N"}.get(ic)
-             or "Sigma=1+8 real Y_lm (l<=16, amplitude 0.05), Pi=0.1, at rest")
+    state = STATES[ic]
     sources = ("accretion on (rate 1e-3, e_acc 0.1, omega_acc 0.8, tilt 0.3, width 0.1), friction alpha 1e-6"
                if acc else "accretion off")
     line = {

@@ -176,3 +176,41 @@ def test_ensemble_autotuned_equals_single_runs():
         s = S.MUSCL_HLLCplus(m, U0, 5, g, om, p is not None, params=p, precision=S.MIXED)
         s.do_steps(d, 20)
         assert np.array_equal(s.get_state(), ens.get_state(r))
+
+
+def test_c4_full_ensemble():
+    """C4 itself: 64 icosphere_hex runs of levels 4-7 (2.67M faces), HLLC+,
+    500 steps in one ensemble.  Every run stays within 1e-5 relative L1
+    density error of the same run advanced alone on the FP64 path, one run
+    of each level equals a single FP32-flux solver bit for bit, and every
+    run's time and stop_check match its FP64 run's."""
+    import splash_amd as S
+    import workloads as W
+    runs = W.c4_runs(S)
+    p = S.default_params(setup_threads=16)
+    ens = S.Ensemble([(m, U0, GAM, 0.0, p) for m, U0, _ in runs], 5, S.HLLCPLUS)
+    n_runs, faces, _, _ = ens.info()
+    assert n_runs == 64 and faces == sum(m.n_faces for m, _, _ in runs)
+    dt0 = [1e-4] * n_runs
+    ens.do_steps(dt0, 500)
+    worst = 0.0
+    for r, (m, U0, lv) in enumerate(runs):
+        f64 = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=p, precision=S.FP64)
+        f64.do_steps(1e-4, 500)
+        A, B = f64.get_state(), ens.get_state(r)
+        l1 = float(np.sum(np.abs(B[:, 0] - A[:, 0])) / np.sum(np.abs(A[:, 0])))
+        worst = max(worst, l1)
+        assert l1 <= 1e-5, (r, lv, l1)
+        assert ens.time(r) == pytest.approx(f64.time(), rel=1e-12)
+        assert ens.get_stop_check(r) == f64.get_stop_check()
+        del f64
+    print(f"C4: worst per-run L1 density error vs FP64 after 500 steps {worst:.3e}")
+    first_of_level = {}
+    for r, (_, _, lv) in enumerate(runs):
+        first_of_level.setdefault(lv, r)
+    for lv, r in sorted(first_of_level.items()):
+        m, U0, _ = runs[r]
+        mx = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=p, precision=S.MIXED)
+        mx.do_steps(1e-4, 500)
+        assert np.array_equal(mx.get_state(), ens.get_state(r)), (r, lv)
+        assert mx.time() == ens.time(r)
