@@ -16,6 +16,9 @@
 // per-step diagnostics line (set_verbose(false) silences it).
 //
 // `threads` (the OpenMP team size of the reference) is accepted and ignored.
+// The constructors take this header's SurfaceMesh or, unchanged, an existing
+// pmp::SurfaceMesh (any type with pmp's vertices() / faces() / position() /
+// vertices(face) interface, flattened by from_pmp()).
 // Extensions: get_state(), do_steps(dt, n), set_verbose(), precision modes,
 // write_checkpoint() / read_checkpoint().
 #pragma once
@@ -28,13 +31,15 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../splash.h"
 
 namespace splash {
 
-constexpr char kCheckpointTag[9] = "SPLASHC1";
+constexpr char kCheckpointTag[9] = "SPLASHC2";
 
 inline void check_rc(int rc) {
     if (rc != SPLASH_OK) throw std::runtime_error(std::string("splash: ") + splash_last_error_message());
@@ -53,6 +58,23 @@ public:
     const std::vector<int32_t>& face_vertices() const { return fv_; }
     splash_mesh_desc desc() const {
         return splash_mesh_desc{int32_t(n_vertices()), int32_t(n_faces()), xyz_.data(), foff_.data(), fv_.data()};
+    }
+    // 64-bit FNV-1a hash of the vertex coordinates' bytes, the face offsets
+    // and the face vertex lists: a checkpoint records it and only a mesh with
+    // the same numbering and coordinates can read it back
+    uint64_t fingerprint() const {
+        uint64_t h = 0xcbf29ce484222325ull;
+        auto mix = [&h](const void* p, size_t n) {
+            const unsigned char* b = static_cast<const unsigned char*>(p);
+            for (size_t i = 0; i < n; ++i) {
+                h ^= b[i];
+                h *= 0x100000001b3ull;
+            }
+        };
+        mix(xyz_.data(), xyz_.size() * sizeof(double));
+        mix(foff_.data(), foff_.size() * sizeof(int32_t));
+        mix(fv_.data(), fv_.size() * sizeof(int32_t));
+        return h;
     }
 
     static SurfaceMesh from_generator(int rc, splash_mesh* m) {
@@ -91,6 +113,35 @@ inline SurfaceMesh quad_sphere(size_t n) {
     splash_mesh* m = nullptr;
     int rc = splash_mesh_quad_sphere(int(n), &m);
     return SurfaceMesh::from_generator(rc, m);
+}
+
+// True for types with pmp::SurfaceMesh's interface (src/pmp/surface_mesh.h:
+// vertices(), faces(), position(Vertex), vertices(Face), Vertex::idx()).
+template <class M, class = void>
+struct is_pmp_mesh : std::false_type {};
+template <class M>
+struct is_pmp_mesh<M, std::void_t<decltype(std::declval<const M&>().position(*std::declval<const M&>().vertices().begin())),
+                                  decltype((*std::declval<const M&>().vertices(*std::declval<const M&>().faces().begin()).begin()).idx())>>
+    : std::true_type {};
+
+// Flattens a pmp::SurfaceMesh as MUSCL_geometry.hpp:66-88 reads it: the
+// vertices' positions in index order, and per face its vertex indices from
+// the vertices-around-face circulator.  The mesh must hold no deleted
+// elements (pmp's generators never leave any).
+template <class PmpMesh, std::enable_if_t<is_pmp_mesh<PmpMesh>::value, int> = 0>
+SurfaceMesh from_pmp(const PmpMesh& mesh) {
+    std::vector<double> xyz;
+    std::vector<int32_t> foff{0}, fv;
+    for (auto v : mesh.vertices()) {
+        if (size_t(v.idx()) != xyz.size() / 3) throw std::runtime_error("pmp mesh has deleted vertices");
+        const auto& p = mesh.position(v);
+        xyz.insert(xyz.end(), {double(p[0]), double(p[1]), double(p[2])});
+    }
+    for (auto f : mesh.faces()) {
+        for (auto v : mesh.vertices(f)) fv.push_back(int32_t(v.idx()));
+        foff.push_back(int32_t(fv.size()));
+    }
+    return SurfaceMesh(std::move(xyz), std::move(foff), std::move(fv));
 }
 
 // The flat keys of input/parameters.json the adiabatic constructor reads
@@ -147,11 +198,21 @@ public:
     // MUSCL_base.hpp:97-225, including the per-step diagnostics line (:213-214)
     void do_step(double dt0) {
         check_rc(splash_do_step(ctx_, dt0));
+        if (steps_ == 0) set_baseline();  // MUSCL_base.hpp:207-210: the first step sets rho_full
         if (verbose_) print_step_line();
         ++steps_;
     }
-    // n steps without per-step host work (HIP-graph replay); prints nothing
+    // n steps without per-step host work (HIP-graph replay); prints nothing.
+    // A run's first step is taken on its own so the mass and energy after it
+    // become the baseline of the diagnostics line, as in do_step.
     void do_steps(double dt0, int n) {
+        if (n <= 0) return;
+        if (steps_ == 0) {
+            check_rc(splash_do_step(ctx_, dt0));
+            set_baseline();
+            ++steps_;
+            --n;
+        }
         check_rc(splash_do_steps(ctx_, dt0, n));
         steps_ += size_t(n);
     }
@@ -178,40 +239,50 @@ public:
     }
 
     // Lossless binary checkpoint, in place of restarting from the text of
-    // write_final_state: an 8-byte tag, n_faces and dim (int64), the time, the
-    // stop_check bits (int64) and the state as raw FP64 in mesh face order.
+    // write_final_state: an 8-byte tag; n_faces, dim, the stop_check bits and
+    // the step count (int64); the mesh fingerprint (uint64); the time and the
+    // diagnostics line's mass and energy baselines (FP64); then the state as
+    // raw FP64 in mesh face order.
     void write_checkpoint(const std::string& path) {
         std::vector<double> flat(mesh_.n_faces() * size_t(dim_));
         check_rc(splash_get_state(ctx_, flat.data()));
         int bits = 0;
         uint64_t first = 0;
         check_rc(splash_stop_bits(ctx_, &bits, &first));
-        const double t = time();
+        const double scalars[3] = {time(), rho_full_, E_full_};
+        const int64_t head[4] = {int64_t(mesh_.n_faces()), int64_t(dim_), int64_t(bits), int64_t(steps_)};
+        const uint64_t print = mesh_.fingerprint();
         std::ofstream out(path, std::ios::binary);
-        const int64_t head[4] = {int64_t(mesh_.n_faces()), int64_t(dim_), int64_t(bits), 0};
         out.write(kCheckpointTag, 8);
         out.write(reinterpret_cast<const char*>(head), sizeof(head));
-        out.write(reinterpret_cast<const char*>(&t), sizeof(t));
+        out.write(reinterpret_cast<const char*>(&print), sizeof(print));
+        out.write(reinterpret_cast<const char*>(scalars), sizeof(scalars));
         out.write(reinterpret_cast<const char*>(flat.data()), std::streamsize(flat.size() * sizeof(double)));
         if (!out) throw std::runtime_error("cannot write checkpoint " + path);
     }
-    // Restores a checkpoint of the same mesh and dim (throws otherwise).
+    // Restores a checkpoint written on the same mesh (same fingerprint) and
+    // dim; throws otherwise.
     void read_checkpoint(const std::string& path) {
         std::ifstream in(path, std::ios::binary);
         char tag[8];
         int64_t head[4];
-        double t = 0;
+        uint64_t print = 0;
+        double scalars[3];
         in.read(tag, 8);
         in.read(reinterpret_cast<char*>(head), sizeof(head));
-        in.read(reinterpret_cast<char*>(&t), sizeof(t));
+        in.read(reinterpret_cast<char*>(&print), sizeof(print));
+        in.read(reinterpret_cast<char*>(scalars), sizeof(scalars));
         if (!in || std::string(tag, 8) != std::string(kCheckpointTag, 8))
             throw std::runtime_error("not a checkpoint: " + path);
-        if (head[0] != int64_t(mesh_.n_faces()) || head[1] != int64_t(dim_))
+        if (head[0] != int64_t(mesh_.n_faces()) || head[1] != int64_t(dim_) || print != mesh_.fingerprint())
             throw std::runtime_error("checkpoint of another mesh or dim: " + path);
         std::vector<double> flat(mesh_.n_faces() * size_t(dim_));
         in.read(reinterpret_cast<char*>(flat.data()), std::streamsize(flat.size() * sizeof(double)));
         if (!in) throw std::runtime_error("truncated checkpoint: " + path);
-        check_rc(splash_set_state(ctx_, flat.data(), t, int(head[2])));
+        check_rc(splash_set_state(ctx_, flat.data(), scalars[0], int(head[2])));
+        steps_ = size_t(head[3]);
+        rho_full_ = scalars[1];
+        E_full_ = scalars[2];
     }
 
     // MUSCL_geometry.hpp:1069-1117
@@ -275,13 +346,16 @@ protected:
 
     static double norm3(const double* p) { return std::sqrt(((0.0 + p[0] * p[0]) + p[1] * p[1]) + p[2] * p[2]); }
 
+    void set_baseline() {
+        double d[5];
+        check_rc(splash_diagnostics(ctx_, d));
+        rho_full_ = d[0];
+        E_full_ = d[4];
+    }
+
     void print_step_line() {
         double d[5];
         check_rc(splash_diagnostics(ctx_, d));
-        if (steps_ == 0) {
-            rho_full_ = d[0];
-            E_full_ = d[4];
-        }
         std::cout << "t= " << time() << " mass_relative_gain=" << d[0] / rho_full_ - 1
                   << " (l/mass)_total_norm= " << std::sqrt(d[1] * d[1] + d[2] * d[2] + d[3] * d[3]) << " l_total = ("
                   << d[1] << "," << d[2] << "," << d[3] << ")" << "\n";
@@ -416,6 +490,11 @@ public:
         : isothermal_gpu(mesh, U_in, dim, gam, SPLASH_HLLE, with_precision(precision)) {
         (void)threads;
     }
+    // from an existing pmp::SurfaceMesh
+    template <class PmpMesh, std::enable_if_t<is_pmp_mesh<PmpMesh>::value, int> = 0>
+    MUSCL_HLLE(const PmpMesh& mesh, std::vector<std::vector<double>> U_in, int dim, double gam, size_t threads = 1,
+               int precision = SPLASH_FP64)
+        : MUSCL_HLLE(from_pmp(mesh), std::move(U_in), dim, gam, threads, precision) {}
 
 private:
     static splash_params with_precision(int precision) {
@@ -434,6 +513,11 @@ private:
             : adiabatic_gpu(mesh, U_in, dim, gam, omega_ns, ID, params_from_json(accretion_on)) {                 \
             (void)threads;                                                                                         \
         }                                                                                                          \
+        /* from an existing pmp::SurfaceMesh */                                                                    \
+        template <class PmpMesh, std::enable_if_t<is_pmp_mesh<PmpMesh>::value, int> = 0>                          \
+        NAME(const PmpMesh& mesh, std::vector<std::vector<double>> U_in, int dim, double gam, double omega_ns,    \
+             bool accretion_on = false, size_t threads = 1)                                                       \
+            : NAME(from_pmp(mesh), std::move(U_in), dim, gam, omega_ns, accretion_on, threads) {}                  \
         /* explicit parameters instead of input/parameters.json (no file I/O for parameters) */                   \
         NAME(SurfaceMesh mesh, std::vector<std::vector<double>> U_in, int dim, double gam, double omega_ns,       \
              const splash_params& params)                                                                          \

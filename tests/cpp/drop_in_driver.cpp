@@ -1,6 +1,9 @@
 // The reference's driver loop (test.cpp:9-152) written against the drop-in
-// classes of include/splash/MUSCL.hpp: same input files, same writers, same
-// stop logic.  Extra arguments choose the mesh: <kind> <level> [nx ny].
+// classes of include/splash/MUSCL.hpp: the same input files, writers, four
+// light-curve files and stop logic.  Two departures: the parameter file is
+// read by a flat-key reader instead of nlohmann::json (the same keys and
+// values), and the mesh is chosen by arguments <kind> <level> [nx ny]
+// instead of being fixed to quad_sphere(6) in the source.
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -60,14 +63,26 @@ int main(int argc, char** argv) {
     test2.write_t_omega_z();
     test2.write_t_L();
 
-    std::ofstream out_lc("results/lightcurve0.dat");
-    auto lc = test2.get_light_curves();
-    out_lc << test2.time() << " " << lc[0] << "\n";
+    std::ofstream out_lc_0("results/lightcurve0.dat");
+    std::ofstream out_lc_45("results/lightcurve45.dat");
+    std::ofstream out_lc_90("results/lightcurve90.dat");
+    std::ofstream out_lc_180("results/lightcurve180.dat");
+    auto write_light_curves = [&]() {
+        const std::vector<double> lc = test2.get_light_curves();
+        out_lc_0 << test2.time() << " " << lc[0] << "\n";
+        out_lc_0.flush();
+        out_lc_45 << test2.time() << " " << lc[1] << "\n";
+        out_lc_45.flush();
+        out_lc_90 << test2.time() << " " << lc[2] << "\n";
+        out_lc_90.flush();
+        out_lc_180 << test2.time() << " " << lc[3] << "\n";
+        out_lc_180.flush();
+    };
+    write_light_curves();
     size_t steps = 0;
     while (test2.time() < t_max && steps < maxstep) {
         test2.do_step(dt);
-        lc = test2.get_light_curves();
-        out_lc << test2.time() << " " << lc[0] << "\n";
+        write_light_curves();
         if (steps % skipstep == 0) {
             test2.write_t_rho();
             test2.write_t_p();

@@ -71,12 +71,81 @@ def test_driver_loop_matches_oracle(tmp_path):
     assert np.allclose(curl, want, rtol=1e-5, atol=1e-5 * np.max(np.abs(want[:, 1:])))
     fc = np.loadtxt(tmp_path / "results" / "face_centers.dat")
     assert fc.shape == (mesh.n_faces, 3) and np.allclose(np.linalg.norm(fc, axis=1), 1, atol=1e-14)
-    # the light curve written every step (get_light_curves on the device)
-    lc = np.loadtxt(tmp_path / "results" / "lightcurve0.dat")
-    assert lc.shape == (1 + PARAMS["maxstep"], 2)
+    # the four light curves written every step (get_light_curves on the
+    # device), as test.cpp:110-122
     g = S.Geometry(mesh)
-    want = light_curves(ref.state(), g.fc, g.area, GAM)[0]
-    assert lc[-1, 0] == pytest.approx(ref.time(), rel=1e-5) and lc[-1, 1] == pytest.approx(want, rel=1e-5)
+    want = light_curves(ref.state(), g.fc, g.area, GAM)
+    for k, name in enumerate(("lightcurve0", "lightcurve45", "lightcurve90", "lightcurve180")):
+        lc = np.loadtxt(tmp_path / "results" / f"{name}.dat")
+        assert lc.shape == (1 + PARAMS["maxstep"], 2)
+        assert lc[-1, 0] == pytest.approx(ref.time(), rel=1e-5) and lc[-1, 1] == pytest.approx(want[k], rel=1e-5)
+    # the writers' other series against values recomputed from the oracle's
+    # states at the same steps (initial state, then after steps 1, 6, 11, 16)
+    ref2 = O.Solver(O.Mesh("quad", 4), np.loadtxt(tmp_path / "input" / "input.dat"), 5, O.HLLCPLUS, GAM, 0.0)
+    states, times = [ref2.state()], [0.0]
+    for k in range(PARAMS["maxstep"]):
+        ref2.do_step(PARAMS["dt"])
+        if k % PARAMS["skipstep"] == 0:
+            states.append(ref2.state())
+            times.append(ref2.time())
+    fc = g.fc
+    r = fc / np.linalg.norm(fc, axis=1)[:, None]
+    theta = np.arccos(fc[:, 2] / np.linalg.norm(fc, axis=1))
+
+    def rows(fn):
+        return np.array([np.concatenate([[t], fn(U)]) for t, U in zip(times, states)])
+
+    def vel(U):
+        return np.cross(r, U[:, 1:4]) / (-U[:, :1])
+
+    def pressure(U):  # adiabatic.hpp:113-133 with :582-592
+        v = vel(U)
+        return np.maximum(1e-16, (U[:, 4] - U[:, 0] * np.sum(v * v, axis=1) / 2) * (GAM - 1))
+
+    def omega_z(U):  # adiabatic.hpp:195-226
+        with np.errstate(divide="ignore", invalid="ignore"):
+            om = np.cross(r, vel(U))[:, 2] / (np.sin(theta) * np.sin(theta))
+        om[~np.isfinite(om) | (np.abs(om) > 1e4)] = 0
+        return om
+
+    series = {"p": pressure, "omega": omega_z, "Lx": lambda U: U[:, 1], "Ly": lambda U: U[:, 2],
+              "Lz": lambda U: U[:, 3], "rho": lambda U: U[:, 0]}
+    for name, fn in series.items():
+        got = np.loadtxt(tmp_path / "results" / f"{name}.dat")
+        want = rows(fn)
+        assert got.shape == want.shape, name
+        scale = np.max(np.abs(want[:, 1:]))
+        assert np.allclose(got[:, 1:], want[:, 1:], rtol=1e-5, atol=1e-5 * scale), name
+        assert np.allclose(got[:, 0], want[:, 0], rtol=1e-5, atol=1e-12), name
+    # the mesh writers (MUSCL_geometry.hpp:1069-1117)
+    faces = [ln.split() for ln in (tmp_path / "results" / "faces.dat").read_text().splitlines()]
+    fo, fv = mesh.face_offsets, mesh.face_vertices
+    assert [[int(x) for x in f] for f in faces] == [list(map(int, fv[fo[f]:fo[f + 1]])) for f in range(mesh.n_faces)]
+    verts = np.loadtxt(tmp_path / "results" / "vertices.dat")
+    assert np.allclose(verts, mesh.vertices, rtol=0, atol=1e-14)
+
+
+def test_pmp_mesh_flattening():
+    """from_pmp() (include/splash/MUSCL.hpp) turns an object with pmp's
+    SurfaceMesh interface into exactly the flat arrays of the same mesh."""
+    prog = os.path.join(ROOT, "tests", "cpp", "pmp_adapter")
+    r = subprocess.run([prog, "flatten"], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_pmp_mesh_constructors(tmp_path):
+    """The drop-in constructors called with a pmp-style mesh step exactly as
+    when called with the flattened mesh (HLLC+ from input/parameters.json,
+    and HLLE)."""
+    (tmp_path / "input").mkdir()
+    (tmp_path / "results").mkdir()
+    (tmp_path / "input" / "parameters.json").write_text(json.dumps(PARAMS, indent=1))
+    prog = os.path.join(ROOT, "tests", "cpp", "pmp_adapter")
+    r = subprocess.run([prog, "solve"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
 
 
 @pytest.mark.gpu
