@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsplash_amd.so")
+# SPLASH_AMD_LIB selects another build of the same library (for example the
+# sanitizer build of the host code); the default is the in-tree build
+LIB_PATH = os.environ.get("SPLASH_AMD_LIB") or os.path.join(HERE, "libsplash_amd.so")
 
 SPLASH_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EMESH", -3: "EDEVICE", -4: "ENOMEM", -5: "EINTERNAL"}

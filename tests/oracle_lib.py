@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_SO = os.environ.get("SPLASH_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 HLLE, HLLE_P, HLLC, HLLCPLUS = 0, 1, 2, 3
 
