@@ -288,6 +288,8 @@ void Engine::capture_graph() {
     }
     check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
     check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    // upload now (stream-ordered) so the first replay does not pay for it
+    check(hipGraphUpload(graph_exec_, stream_), "hipGraphUpload");
 }
 
 void Engine::steps(double dt0, int n) {

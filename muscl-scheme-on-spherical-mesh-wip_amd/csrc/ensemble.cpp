@@ -280,6 +280,8 @@ void Ensemble::capture_graph() {
     }
     check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
     check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    // upload now (stream-ordered) so the first replay does not pay for it
+    check(hipGraphUpload(graph_exec_, stream_), "hipGraphUpload");
 }
 
 void Ensemble::steps(const double* dt0, int n) {
