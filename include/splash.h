@@ -153,6 +153,23 @@ int splash_diagnostics(splash_ctx *ctx, double out[5]);
  * transfer), made by the first of them after a step and reused by the
  * others until the next step or splash_set_state. */
 int splash_light_curves(splash_ctx *ctx, double out[4]);
+/* Allocation-free readout.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under their own HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Enqueues on
+ * `stream` (a hipStream_t; NULL = the context's own stream, after its
+ * steps) the readout of the context's current state and returns without
+ * waiting: d_out, device memory of 9 doubles, receives {sum Sigma*A,
+ * sum Lx*A, sum Ly*A, sum Lz*A, sum E*A, flux_0, flux_45, flux_90,
+ * flux_180} (light curves 0 for dim 4), bit for bit the values of
+ * splash_diagnostics and splash_light_curves.  It calls no allocator: its
+ * per-workgroup partial sums live in a fixed array of 288 KiB that each GPU
+ * holds from the first call until the process ends, which limits it to
+ * SPLASH_READOUT_NOALLOC_MAX_FACES faces (SPLASH_EINVAL beyond).  Every call
+ * on one GPU shares that array, so calls that overlap in time on one GPU
+ * race; a caller passing another stream orders it after the context's steps
+ * (splash_synchronize, or its own event). */
+#define SPLASH_READOUT_NOALLOC_MAX_FACES 4194304
+int splash_readout_noalloc(splash_ctx *ctx, double *d_out, void *stream);
 /* write_t_curl's per-face values (adiabatic.hpp:135-178, isothermal.hpp:
  * 72-110), out[n_faces] in mesh face order: for face i,
  * sum_j ((ec_j x L+_j) / -Sigma_i) . (v_j - v_j1) / A_i, with L+ the face's

@@ -280,6 +280,15 @@ int splash_light_curves(splash_ctx* ctx, double out[4]) {
     });
 }
 
+int splash_readout_noalloc(splash_ctx* ctx, double* d_out, void* stream) {
+    CTX_OR_FAIL;
+    if (!d_out) return fail(SPLASH_EINVAL, "null output");
+    return guarded([&] {
+        ctx->engine->readout_noalloc(d_out, static_cast<hipStream_t>(stream));
+        return SPLASH_OK;
+    });
+}
+
 int splash_curl(splash_ctx* ctx, double* out) {
     CTX_OR_FAIL;
     if (!out) return fail(SPLASH_EINVAL, "null output");

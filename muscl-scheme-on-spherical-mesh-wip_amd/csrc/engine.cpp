@@ -51,6 +51,7 @@ void Engine::init(const RunHost& r, const double* U_in) {
     nf_ = r.geo.nf;
     nh_ = r.geo.nh;
     const int32_t nf = r.geo.nf;
+    check(hipGetDevice(&device_), "hipGetDevice");
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     try {
         upload(r);
@@ -388,6 +389,16 @@ void Engine::light_curves(double out[4]) {
     if (dim_ != 5) throw std::invalid_argument("light curves exist for the adiabatic (dim 5) solvers only");
     ensure_readout();
     for (int k = 0; k < 4; ++k) out[k] = h_readout_->sums[5 + k];
+}
+
+void Engine::readout_noalloc(double* d_out, hipStream_t s) {
+    if (!d_out) throw std::invalid_argument("d_out is null");
+    if (nf_ > fp64::readout_noalloc_max_faces())
+        throw std::invalid_argument("the allocation-free readout takes at most " +
+                                    std::to_string(fp64::readout_noalloc_max_faces()) + " faces, this run has " +
+                                    std::to_string(nf_));
+    check(fp64::enqueue_readout_noalloc(args_, d_light_mask_, d_light_obs_, device_, d_out, s ? s : stream_),
+          "allocation-free readout");
 }
 
 void Engine::curl(double* out) {

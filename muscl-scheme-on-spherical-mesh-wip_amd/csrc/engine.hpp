@@ -61,6 +61,9 @@ public:
     void set_state(const double* U_in, double t, int stop_bits);
     void diagnostics(double out[5]);    // sum of (Sigma, L, E) * area
     void light_curves(double out[4]);   // get_light_curves (dim 5)
+    // the same nine sums into device memory d_out[9] on stream s (null: the
+    // run's stream), enqueued without waiting and without allocating
+    void readout_noalloc(double* d_out, hipStream_t s);
     // write_t_curl's per-face values [nf] (mesh order) of the current state;
     // needs an Engine built from a mesh
     void curl(double* out);
@@ -104,6 +107,7 @@ private:
     std::vector<int32_t> he_old_;            // internal -> mesh half-edge
 
     hipStream_t stream_ = nullptr;
+    int device_ = 0;                    // the GPU the run lives on
     std::vector<void*> allocations_;
     StepArgs args_{};
     uint8_t* d_light_mask_ = nullptr;   // [nf] light-curve observer visibility bits (dim 5)

@@ -55,6 +55,14 @@ void enqueue_readout_runs(const StepArgs& a, const uint8_t* light_mask, const do
 void enqueue_curl(const StepArgs& a, bool stepped, const double* ev, double* W_tmp, double* out, hipStream_t s);
 int block_size();
 int finalize_blocks();  // dt-kernel workgroups per run
+// enqueue_readout with its per-workgroup partials (and the scalars copy) in
+// the library's fixed per-GPU array instead of a caller buffer: out is a
+// device array of 9 doubles.  Allocates nothing; returns
+// hipErrorInvalidValue for a mesh of more than readout_noalloc_max_faces()
+// faces.  Calls that overlap on one GPU share the array and race.
+int readout_noalloc_max_faces();
+hipError_t enqueue_readout_noalloc(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, int device,
+                                   double* out, hipStream_t s);
 // *p = v, ordered on stream s (a one-thread kernel: no memset node in a graph)
 void enqueue_set_scalar(double* p, double v, hipStream_t s);
 }  // namespace fp64

@@ -4,6 +4,9 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
+#include "../../include/splash.h"
 #include "engine_types.hpp"
 #include "launch.hpp"
 
@@ -90,6 +93,57 @@ __global__ void k_set_scalar(double* p, double v) {
 }
 
 void enqueue_set_scalar(double* p, double v, hipStream_t s) { k_set_scalar<<<1, 64, 0, s>>>(p, v); }
+
+// ---- the allocation-free readout ----
+// The per-workgroup partial sums of k_readout_partial, followed by the copy
+// of the scalars that k_readout_final writes, in one fixed array per GPU
+// (each GPU that loads the library holds its copy until the process ends):
+// kScratchBlocks * kReadout + 8 doubles, 288 KiB.
+constexpr int kScratchBlocks = 4096;
+constexpr size_t kScratchDoubles = size_t(kScratchBlocks) * kReadout + 8;
+static_assert(sizeof(Scalars) <= 8 * sizeof(double), "the scalars copy fits its slot");
+static_assert(kScratchBlocks * kReadoutFaces == SPLASH_READOUT_NOALLOC_MAX_FACES, "the limit include/splash.h states");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+}  // namespace
+
+// The device address of `device`'s copy of scratch, looked up once per
+// device with that device current (the host-side name of a __device__ array
+// is no device address), then kept.
+static hipError_t scratch_address(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
+int readout_noalloc_max_faces() { return kScratchBlocks * kReadoutFaces; }
+
+hipError_t enqueue_readout_noalloc(const StepArgs& a, const uint8_t* light_mask, const double* light_obs, int device,
+                                   double* out, hipStream_t s) {
+    if (readout_blocks(a.geom.nf) > kScratchBlocks) return hipErrorInvalidValue;
+    double* part = nullptr;
+    const hipError_t e = scratch_address(device, &part);
+    if (e != hipSuccess) return e;
+    Scalars* sc_copy = reinterpret_cast<Scalars*>(part + size_t(kScratchBlocks) * kReadout);
+    enqueue_readout(a, light_mask, light_obs, part, out, sc_copy, s);
+    return hipGetLastError();
+}
 
 int block_size() { return kBlock; }
 int finalize_blocks() { return kFinalizeBlocks; }
