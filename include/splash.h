@@ -124,6 +124,12 @@ int splash_do_steps(splash_ctx *ctx, double dt0, int n);
  * to its dispatch, one wait at the end; ms_out[4] receives the mean
  * duration in ms of [dt finalize, stage 1, stage 2, whole step]. */
 int splash_profile_step(splash_ctx *ctx, double dt0, int n, double *ms_out);
+/* Measurement extension (no counterpart in the reference): the HBM
+ * bandwidth the library's own streaming copy kernels reach on the current
+ * GPU, copying `bytes` (>= 1 MiB) `reps` times with 16-byte vector loads and
+ * stores, plain and non-temporal; *gbs_out = the faster form's (read +
+ * write) bytes per second / 1e9.  Allocates and frees its two buffers. */
+int splash_copy_bandwidth(int64_t bytes, int reps, double *gbs_out);
 /* Waits for every enqueued step. */
 int splash_synchronize(splash_ctx *ctx);
 

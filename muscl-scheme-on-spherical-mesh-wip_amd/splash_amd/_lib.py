@@ -101,6 +101,7 @@ SIGNATURES = {
     "splash_ensemble_diagnostics": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_ensemble_light_curves": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_readout_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "splash_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "splash_last_error_message": (C.c_char_p, []),
 }
 
