@@ -165,7 +165,7 @@ def ensemble_bench(S, steps, warmup, dt0, profile_steps):
     ens = S.Ensemble([(m, U0, GAM, 0.0, params(S, False, patch_faces=-1)) for m, U0, _ in runs], 5, S.HLLCPLUS)
     t_setup = time.perf_counter() - t_setup
     n_runs, F, H, blocks = ens.info()
-    tuned_faces = ens.patch_faces()
+    tuned_faces, tuned_threads = ens.patch_shape()
     ens.do_steps(dt0, warmup)
     ens.synchronize()
     warm_device(S)
@@ -213,7 +213,7 @@ def ensemble_bench(S, steps, warmup, dt0, profile_steps):
         "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
         "stage_2_hbm_frac": ab / (prof[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "workgroups_per_stage": blocks, "runs_with_stop_check": int(stops),
-        "patch_faces_tuned": tuned_faces, "setup_s_incl_tuning": t_setup,
+        "patch_faces_tuned": tuned_faces, "block_threads_tuned": tuned_threads, "setup_s_incl_tuning": t_setup,
     }
 
 

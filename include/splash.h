@@ -79,9 +79,13 @@ typedef struct splash_params {
     int32_t setup_threads; /* host threads for the geometry precompute; 0 = every host thread
                             * (OpenMP's default team); the tables do not depend on it */
     int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..64; 0 = 64;
-                          * -1 = the fastest of a few candidates, measured at creation */
+                          * -1 = the fastest of a few (block_threads, patch_faces) candidates,
+                          * measured at creation */
     int32_t no_graph;    /* 0: splash_do_steps replays a captured HIP graph (captured after the
                           * first call's steps); 1: every step is launched on its own */
+    int32_t block_threads; /* SPLASH_MIXED: threads per stage-kernel workgroup, 128, 192 or 256
+                            * (0 = 256); a patch's half-edges plus halo fit 2 * block_threads
+                            * LDS slots; ignored when patch_faces = -1 (the tuner picks it) */
 } splash_params;
 
 typedef struct splash_mesh splash_mesh;   /* library-owned generated mesh */
@@ -186,6 +190,10 @@ int splash_num_faces(const splash_ctx *ctx, int32_t *n_faces, int32_t *n_halfedg
 /* faces per stage-kernel patch in use (SPLASH_MIXED; the tuned value when
  * params.patch_faces = -1) */
 int splash_patch_faces(const splash_ctx *ctx, int32_t *patch_faces);
+/* the stage kernel's patch shape in use: faces per patch and threads per
+ * workgroup (SPLASH_MIXED; both 0 for SPLASH_FP64; the tuned values when
+ * params.patch_faces = -1) */
+int splash_patch_shape(const splash_ctx *ctx, int32_t *patch_faces, int32_t *block_threads);
 /* Face centres (MUSCL_geometry.hpp:176-187) and areas (:989-1030). */
 int splash_face_geometry(const splash_ctx *ctx, double *centers_out, double *areas_out);
 
@@ -247,6 +255,9 @@ int splash_ensemble_info(const splash_ensemble *e, int32_t *n_runs, int64_t *tot
 int splash_ensemble_num_faces(const splash_ensemble *e, int run, int32_t *n_faces);
 /* faces per patch in use (the tuned value when params[0].patch_faces = -1) */
 int splash_ensemble_patch_faces(const splash_ensemble *e, int32_t *patch_faces);
+/* faces per patch and threads per workgroup in use (params[0] sets both, or
+ * the tuner when params[0].patch_faces = -1) */
+int splash_ensemble_patch_shape(const splash_ensemble *e, int32_t *patch_faces, int32_t *block_threads);
 int splash_ensemble_time(splash_ensemble *e, int run, double *t);
 int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
 int splash_ensemble_stop_check(splash_ensemble *e, int run, int *stop);

@@ -116,6 +116,7 @@ void splash_default_params(splash_params* p) {
     p->setup_threads = 0;
     p->patch_faces = 0;
     p->no_graph = 0;
+    p->block_threads = 0;
 }
 
 int splash_mesh_uv_sphere(int n_slices, int n_stacks, splash_mesh** out) {
@@ -155,7 +156,7 @@ int splash_create(const splash_mesh_desc* mesh, const double* U_in, int dim, int
         splash::PolyMesh m = from_desc(mesh);
         auto ctx = std::make_unique<splash_ctx>();
         ctx->engine = std::make_unique<splash::Engine>(m, U_in, dim, solver, gam, omega_ns, rp, p.precision,
-                                                       p.setup_threads, p.patch_faces);
+                                                       p.setup_threads, p.patch_faces, p.block_threads);
         ctx->engine->set_use_graph(!p.no_graph);
         *out = ctx.release();
         return SPLASH_OK;
@@ -298,6 +299,15 @@ int splash_curl(splash_ctx* ctx, double* out) {
     });
 }
 
+int splash_patch_shape(const splash_ctx* ctx, int32_t* patch_faces, int32_t* block_threads) {
+    CTX_OR_FAIL;
+    if (!patch_faces || !block_threads) return fail(SPLASH_EINVAL, "null output");
+    const bool mixed = ctx->engine->precision() == SPLASH_MIXED;
+    *patch_faces = mixed ? ctx->engine->patch_faces() : 0;
+    *block_threads = mixed ? ctx->engine->block_threads() : 0;
+    return SPLASH_OK;
+}
+
 int splash_patch_faces(const splash_ctx* ctx, int32_t* patch_faces) {
     CTX_OR_FAIL;
     if (!patch_faces) return fail(SPLASH_EINVAL, "null output");
@@ -353,7 +363,8 @@ int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
     if (!out) return fail(SPLASH_EINVAL, "null output");
     return guarded([&] {
         const splash::Geometry g = splash::build_geometry(from_desc(mesh), 0);
-        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(), splash::mixed::max_faces());
+        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(splash::mixed::default_threads()),
+                                                     splash::mixed::max_faces());
         splash::validate_layout(g, L);
         int32_t max_halo = 0, nonempty = 0;
         for (int32_t b = 0; b < L.n_blocks(); ++b) {
@@ -388,7 +399,7 @@ int splash_ensemble_create(int n_runs, const splash_mesh_desc* meshes, const dou
         }
         auto e = std::make_unique<splash_ensemble>();
         e->ens = std::make_unique<splash::Ensemble>(runs, dim, solver, params[0].setup_threads,
-                                                    params[0].patch_faces);
+                                                    params[0].patch_faces, params[0].block_threads);
         e->ens->set_use_graph(!params[0].no_graph);
         *out = e.release();
         return SPLASH_OK;
@@ -443,6 +454,14 @@ int splash_ensemble_num_faces(const splash_ensemble* e, int run, int32_t* n_face
     RUN_OR_FAIL;
     if (!n_faces) return fail(SPLASH_EINVAL, "null output");
     *n_faces = e->ens->n_faces(run);
+    return SPLASH_OK;
+}
+
+int splash_ensemble_patch_shape(const splash_ensemble* e, int32_t* patch_faces, int32_t* block_threads) {
+    ENS_OR_FAIL;
+    if (!patch_faces || !block_threads) return fail(SPLASH_EINVAL, "null output");
+    *patch_faces = e->ens->patch_faces();
+    *block_threads = e->ens->block_threads();
     return SPLASH_OK;
 }
 

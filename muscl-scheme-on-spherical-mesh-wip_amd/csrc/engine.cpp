@@ -22,17 +22,20 @@ void check(hipError_t e, const char* what) {
 }  // namespace
 
 Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-               const RunParams& params, int precision, int geometry_threads, int patch_faces)
+               const RunParams& params, int precision, int geometry_threads, int patch_faces, int block_threads)
     : dim_(dim), solver_(solver), precision_(precision), gam_(gam), omega_ns_(omega_ns), params_(params) {
     if (precision != kFP64 && precision != kMixed) throw std::invalid_argument("unknown precision mode");
     if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
+    if (block_threads != 0 && !mixed::valid_threads(block_threads))
+        throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
     const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
+    const int32_t threads = block_threads ? block_threads : mixed::default_threads();
     RunHost r = precision == kMixed
-                    ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(), cap,
+                    ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(threads), cap,
                                      geometry_threads)
                     : build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, fp64::block_size(),
                                      fp64::block_size(), geometry_threads);
-    if (precision == kMixed && patch_faces == -1) autotune_patch_faces(r, U_in, 1e-4);
+    if (precision == kMixed && patch_faces == -1) autotune_patch_shape(r, U_in, 1e-4);
     init(r, U_in);
     geo_ = std::move(r.geo);
 }
@@ -48,6 +51,7 @@ void Engine::init(const RunHost& r, const double* U_in) {
     area_coeff_ = r.area_coeff;
     total_mass_ = r.total_mass;
     patch_faces_ = r.L.max_faces;
+    block_threads_ = precision_ == kMixed ? r.L.slots / 2 : fp64::block_size();
     nf_ = r.geo.nf;
     nh_ = r.geo.nh;
     const int32_t nf = r.geo.nf;
@@ -55,6 +59,7 @@ void Engine::init(const RunHost& r, const double* U_in) {
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     try {
         upload(r);
+        args_.block_threads = block_threads_;
         std::vector<double> U(size_t(nf) * dim_);
         for (int32_t k = 0; k < nf; ++k)
             for (int c = 0; c < dim_; ++c) U[size_t(k) * dim_ + c] = U_in[size_t(perm_[k]) * dim_ + c];
@@ -77,11 +82,11 @@ void Engine::init(const RunHost& r, const double* U_in) {
     }
 }
 
-int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0) {
-    int32_t best = kPatchFacesMax;
+PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0) {
+    PatchShape best{mixed::default_threads(), kPatchFacesMax};
     double best_t = 0;
-    for (int32_t cap : kPatchFaceCandidates) {
-        relayout(r, mixed::slots(), cap);
+    for (const PatchShape& c : kPatchShapeCandidates) {
+        relayout(r, mixed::slots(c.threads), c.faces);
         Engine e(r, U_in, kMixed);
         e.steps(dt0, 16);
         e.synchronize();
@@ -91,10 +96,11 @@ int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0) {
         const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (best_t == 0 || t < best_t) {
             best_t = t;
-            best = cap;
+            best = c;
         }
     }
-    if (r.L.max_faces != best) relayout(r, mixed::slots(), best);
+    if (r.L.max_faces != best.faces || r.L.slots != mixed::slots(best.threads))
+        relayout(r, mixed::slots(best.threads), best.faces);
     return best;
 }
 

@@ -26,10 +26,13 @@ public:
 class Engine {
 public:
     // U_in is [nf][dim] in mesh face order (slot 4 = energy for dim 5).
-    // patch_faces (mixed precision): faces per patch of the stage kernel,
-    // 1..64, 0 = 64, -1 = the fastest of a few measured candidates
+    // Mixed precision: patch_faces = faces per patch of the stage kernel,
+    // 1..64 (0 = 64), block_threads = threads per stage workgroup, 128, 192
+    // or 256 (0 = 256), and patch_faces = -1 takes the fastest of a few
+    // measured (threads, faces) candidates instead
     Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-           const RunParams& params, int precision, int geometry_threads = 0, int patch_faces = 0);
+           const RunParams& params, int precision, int geometry_threads = 0, int patch_faces = 0,
+           int block_threads = 0);
     // from a prepared run, whose layout sets the patch shape
     Engine(const RunHost& r, const double* U_in, int precision);
     ~Engine();
@@ -75,6 +78,7 @@ public:
     double area_coeff() const { return area_coeff_; }
     double total_mass() const { return total_mass_; }
     int32_t patch_faces() const { return patch_faces_; }
+    int32_t block_threads() const { return block_threads_; }
     // the host geometry tables (empty for an Engine built from a RunHost)
     const Geometry& geometry() const { return geo_; }
     hipStream_t stream() const { return stream_; }
@@ -100,6 +104,7 @@ private:
     RunParams params_;
     double area_coeff_ = 0, total_mass_ = 0;
     int32_t patch_faces_ = 0;
+    int32_t block_threads_ = 0;
     int32_t nf_ = 0, nh_ = 0;
 
     // internal layout: faces in bisection or curve order, half-edges face-major
@@ -136,14 +141,21 @@ private:
     bool use_graph_ = true;
 };
 
-// Mixed-precision patch face cap with the shortest measured step: each
+// A patch shape of the mixed kernel: threads per workgroup (two LDS slots
+// each) and the face cap of a patch.
+struct PatchShape {
+    int32_t threads, faces;
+};
+
+// The mixed-precision patch shape with the shortest measured step: each
 // candidate re-lays out r, runs a trial Engine on its own copy of the state
 // (16 graph-replayed warm-up steps, then 32 timed ones at dt0) and r is left
 // laid out for the winner, which is returned.
-int32_t autotune_patch_faces(RunHost& r, const double* U_in, double dt0);
+PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0);
 
 // faces per patch the mixed kernel allows, and the candidates of the tuner
 constexpr int32_t kPatchFacesMax = 64;
-constexpr int32_t kPatchFaceCandidates[] = {64, 48, 40, 32, 24};
+constexpr PatchShape kPatchShapeCandidates[] = {{256, 64}, {256, 48}, {256, 32}, {256, 24},
+                                                {192, 48}, {192, 32}, {128, 32}, {128, 24}};
 
 }  // namespace splash

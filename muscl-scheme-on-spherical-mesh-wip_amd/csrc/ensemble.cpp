@@ -28,27 +28,31 @@ void* Ensemble::dev(size_t bytes, const void* host) {
     return p;
 }
 
-Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads, int patch_faces)
+Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads, int patch_faces,
+                   int block_threads)
     : dim_(dim), solver_(solver) {
     if (runs.empty()) throw std::invalid_argument("an ensemble needs at least one run");
     if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
+    if (block_threads != 0 && !mixed::valid_threads(block_threads))
+        throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
     const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
+    const int32_t threads = block_threads ? block_threads : mixed::default_threads();
     std::vector<RunHost> hosts;
     std::vector<const double*> U_in;
     for (const Run& run : runs) {
         if (!run.mesh) throw std::invalid_argument("null mesh");
         hosts.push_back(build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
-                                       mixed::slots(), cap, geometry_threads));
+                                       mixed::slots(threads), cap, geometry_threads));
         U_in.push_back(run.U_in);
     }
     if (patch_faces == -1) {
-        // the fastest patch cap for the whole ensemble, measured on trial
+        // the fastest patch shape for the whole ensemble, measured on trial
         // ensembles with their own copies of the states
-        int32_t best = kPatchFacesMax;
+        PatchShape best{threads, kPatchFacesMax};
         double best_t = 0;
         std::vector<double> dt0(runs.size(), 1e-4);
-        for (int32_t c : kPatchFaceCandidates) {
-            for (RunHost& h : hosts) relayout(h, mixed::slots(), c);
+        for (const PatchShape& c : kPatchShapeCandidates) {
+            for (RunHost& h : hosts) relayout(h, mixed::slots(c.threads), c.faces);
             Ensemble trial(hosts, U_in, dim, solver);
             trial.steps(dt0.data(), 16);
             trial.synchronize();
@@ -62,7 +66,8 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
             }
         }
         for (RunHost& h : hosts)
-            if (h.L.max_faces != best) relayout(h, mixed::slots(), best);
+            if (h.L.max_faces != best.faces || h.L.slots != mixed::slots(best.threads))
+                relayout(h, mixed::slots(best.threads), best.faces);
     }
     init(hosts, U_in);
 }
@@ -76,6 +81,9 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     const int dim = dim_;
     const int R = int(hosts.size());
     patch_faces_ = hosts[0].L.max_faces;
+    block_threads_ = hosts[0].L.slots / 2;
+    for (const RunHost& h : hosts)
+        if (h.L.slots != hosts[0].L.slots) throw std::invalid_argument("ensemble runs laid out for different blocks");
 
     // merged tables: every run's arrays appended, indices shifted by the
     // faces / half-edges / halo slots / face-set entries before it; only
@@ -216,6 +224,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         const size_t row = dim == 5 ? 5 : 4;  // float4 (Sigma, L) rows, then the pressures
         args_.W32 = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * size_t(F), nullptr));
+        args_.block_threads = block_threads_;
 
         mixed::enqueue_prepare(args_, stream_);
         check(hipGetLastError(), "prepare launch");

@@ -26,7 +26,8 @@ public:
     };
     // patch_faces as for Engine: 1..64, 0 = 64, -1 = the fastest of a few
     // candidates measured on trial ensembles
-    Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads = 0, int patch_faces = 0);
+    Ensemble(const std::vector<Run>& runs, int dim, int solver, int geometry_threads = 0, int patch_faces = 0,
+             int block_threads = 0);
     ~Ensemble();
     Ensemble(const Ensemble&) = delete;
     Ensemble& operator=(const Ensemble&) = delete;
@@ -50,6 +51,7 @@ public:
     int64_t total_halfedges() const { return args_.geom.nh; }
     int32_t n_blocks() const { return args_.geom.n_blocks; }
     int32_t patch_faces() const { return patch_faces_; }
+    int32_t block_threads() const { return block_threads_; }
     double time(int r);
     double last_dt(int r);
     int stop_bits(int r);
@@ -76,6 +78,7 @@ private:
 
     int dim_, solver_;
     int32_t patch_faces_ = 0;
+    int32_t block_threads_ = 0;
     std::vector<int32_t> faces_, face_off_;       // per run
     std::vector<std::vector<int32_t>> perm_;      // per run: internal -> mesh face
     std::vector<double> dt0_host_;

@@ -19,6 +19,7 @@ struct StepArgs {
     double* W2;  // [nf][dim] p-form input of stage 2
     float* W32;  // mixed runs: FP32 copy of the stage input, [nf] float4 (Sigma, L) then (dim 5) [nf] pressure
     float* W32b; // mixed runs: the other stage's FP32 input
+    int block_threads;  // mixed runs: threads per stage-kernel workgroup (128, 192 or 256)
 };
 
 // Optional start/stop events of one launch: bound to the kernel's own
@@ -70,8 +71,10 @@ void enqueue_set_scalar(double* p, double v, hipStream_t s);
 namespace mixed {
 void enqueue_prepare(const StepArgs& a, hipStream_t s);
 void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
-int slots();      // LDS half-edge slots per block
-int max_faces();  // faces per block
+int slots(int threads);  // LDS half-edge slots per block of `threads` threads
+int max_faces();         // faces per block
+int default_threads();   // threads per block unless set or tuned
+bool valid_threads(int threads);
 }  // namespace mixed
 
 }  // namespace splash

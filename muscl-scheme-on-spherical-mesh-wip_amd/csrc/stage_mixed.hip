@@ -35,10 +35,12 @@ __device__ __forceinline__ float m_pow(float x, float y) {
 // input.
 
 // Patches: at most kFaces faces (one wave updates them) whose own
-// half-edges plus halo fit kSlots LDS slots, two slots per thread.
-constexpr int kSlots = 2 * kBlock;
+// half-edges plus halo fit 2 * NT LDS slots, two slots per thread of an
+// NT-thread workgroup.  NT is one of the block shapes below, chosen per run
+// (or ensemble) by the caller or the tuner.
 constexpr int kFaces = 64;
-static_assert(kSlots <= 512, "slot indices are packed in 9 bits");
+constexpr int kThreadChoices[] = {128, 192, 256};
+static_assert(2 * 256 <= 512, "slot indices are packed in 9 bits");
 
 // Element `idx` of a device table through a 32-bit byte offset (each table
 // of the mixed path is below 4 GiB, checked at upload), so an access is a
@@ -401,8 +403,8 @@ __device__ __forceinline__ void reconstruct_slot(const float4* sW, const float* 
                          v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
 }
 
-template <int DIM, int SOLVER, int STAGE>
-__global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysParams* __restrict__ Pv,
+template <int DIM, int SOLVER, int STAGE, int NT>
+__global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams* __restrict__ Pv,
                                                        Scalars* __restrict__ scv, double* __restrict__ U,
                                                        const float* __restrict__ W32,
                                                        float* __restrict__ W32_out) {
@@ -441,21 +443,21 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     SlotRows r0;
     if (t < n_slots) r0 = load_slot(g, ext0 + uint32_t(t), edge_base);
     {
-        // a face set holds at most 1024 faces (4 per thread): every index is
-        // requested before the first gather, so each thread waits for two
+        // a face set holds at most 1024 faces (kPer per thread): every index
+        // is requested before the first gather, so each thread waits for two
         // round trips, not two per face
         const int fs0 = g.fset_off[b], n_fs = g.fset_off[b + 1] - fs0;
         const float4* W4 = reinterpret_cast<const float4*>(W32);
-        constexpr int kPer = 1024 / kBlock;
+        constexpr int kPer = (1024 + NT - 1) / NT;
         uint32_t f[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const int q = t + j * kBlock;
+            const int q = t + j * NT;
             f[j] = q < n_fs ? uint32_t(*row(g.fset_face, uint32_t(fs0 + q))) : 0u;
         }
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const int q = t + j * kBlock;
+            const int q = t + j * NT;
             if (q < n_fs) {
                 const float4 w = *row(W4, f[j]);
                 sW[q] = w;
@@ -471,10 +473,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
     __syncthreads();
 
     // phase 1: reconstruct own and halo half-edges into LDS, slots t and
-    // t + kBlock; thread k then requests the rows of edge k for phase 2
+    // t + NT; thread k then requests the rows of edge k for phase 2
     const int n_edges = n_slots >> 1;
     {
-        const int t1 = t + kBlock;
+        const int t1 = t + NT;
         SlotRows r1;
         if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
         if (t < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r0, &sUP[t * DIM]);
@@ -616,17 +618,27 @@ __global__ __launch_bounds__(kBlock, 5) void k_stage32(GeomPtrs g, const PhysPar
 
 namespace {
 
-template <int DIM, int SOLVER>
-void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
+template <int DIM, int SOLVER, int NT>
+void stage_nt(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
     const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 28 : 20) + sizeof(float) * DIM * size_t(a.geom.slot_max);
-    if (a.geom.fset_max > 1024 || a.geom.slot_max > kSlots || a.geom.slot_max % 4 || lds > 64 * 1024)
+    if (a.geom.fset_max > 1024 || a.geom.slot_max > 2 * NT || a.geom.slot_max % 4 || lds > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
-        launch(k_stage32<DIM, SOLVER, 1>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
-               a.W32, a.W32b);
+        launch(k_stage32<DIM, SOLVER, 1, NT>, a.geom.n_blocks, NT, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U, a.W32,
+               a.W32b);
     else
-        launch(k_stage32<DIM, SOLVER, 2>, a.geom.n_blocks, kBlock, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U,
-               a.W32b, a.W32);
+        launch(k_stage32<DIM, SOLVER, 2, NT>, a.geom.n_blocks, NT, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U, a.W32b,
+               a.W32);
+}
+
+template <int DIM, int SOLVER>
+void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
+    switch (a.block_threads) {
+    case 128: stage_nt<DIM, SOLVER, 128>(a, stage, s, ev); break;
+    case 192: stage_nt<DIM, SOLVER, 192>(a, stage, s, ev); break;
+    case 256: stage_nt<DIM, SOLVER, 256>(a, stage, s, ev); break;
+    default: throw std::invalid_argument("unsupported stage-kernel block size");
+    }
 }
 
 }  // namespace
@@ -670,8 +682,14 @@ void enqueue_prepare(const StepArgs& a, hipStream_t s) {
     else k_prepare32<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W32);
 }
 
-int slots() { return kSlots; }
+int slots(int threads) { return 2 * threads; }
 int max_faces() { return kFaces; }
+int default_threads() { return 256; }
+bool valid_threads(int threads) {
+    for (int c : kThreadChoices)
+        if (c == threads) return true;
+    return false;
+}
 
 void enqueue_stage(const StepArgs& a, int st, hipStream_t s, const KernelEvents* ev) {
     switch (a.phys.solver) {

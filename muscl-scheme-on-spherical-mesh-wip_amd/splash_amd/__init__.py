@@ -244,6 +244,13 @@ class _MUSCL:
         check(lib().splash_patch_faces(self._ctx, C.byref(n)))
         return n.value
 
+    def patch_shape(self):
+        """(faces per patch, threads per workgroup) of the stage kernel
+        (mixed precision; (0, 0) for FP64)."""
+        f, t = C.c_int32(), C.c_int32()
+        check(lib().splash_patch_shape(self._ctx, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
 
 class MUSCL_HLLE(_MUSCL):
     """MUSCL_HLLE(mesh, U_in, dim, gam, threads) -- HLLE.hpp:13, isothermal dim 4."""
@@ -327,6 +334,11 @@ class Ensemble:
         n = C.c_int32()
         check(lib().splash_ensemble_patch_faces(self._e, C.byref(n)))
         return n.value
+
+    def patch_shape(self):
+        f, t = C.c_int32(), C.c_int32()
+        check(lib().splash_ensemble_patch_shape(self._e, C.byref(f), C.byref(t)))
+        return f.value, t.value
 
     def time(self, run):
         t = C.c_double()

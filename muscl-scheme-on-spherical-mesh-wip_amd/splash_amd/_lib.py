@@ -35,7 +35,7 @@ class Params(C.Structure):
                 ("acc_rate", C.c_double), ("e_acc", C.c_double), ("omega_acc_abs", C.c_double),
                 ("tilt_angle", C.c_double), ("acc_width", C.c_double), ("alpha", C.c_double),
                 ("precision", C.c_int32), ("setup_threads", C.c_int32), ("patch_faces", C.c_int32),
-                ("no_graph", C.c_int32)]
+                ("no_graph", C.c_int32), ("block_threads", C.c_int32)]
 
 
 class GeometryView(C.Structure):
@@ -93,6 +93,8 @@ SIGNATURES = {
     "splash_ensemble_num_faces": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32)]),
     "splash_ensemble_patch_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "splash_patch_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "splash_patch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "splash_ensemble_patch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "splash_ensemble_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_ensemble_last_dt": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "splash_ensemble_stop_check": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),

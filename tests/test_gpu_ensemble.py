@@ -142,24 +142,41 @@ def test_ensemble_rejects_mixed_solver_shapes():
         S.Ensemble([(m, U4, GAM, 0.0, None)], 4, S.HLLCPLUS)  # dim 4 with an adiabatic solver
 
 
-def test_patch_shape_and_autotuner_leave_results_unchanged():
-    """Each edge's flux is evaluated from its half-edge of lower mesh index with the
-    same inputs whatever patch holds it, so the state does not depend on the
-    patch face cap: fixed caps and the tuned one agree bit for bit."""
+SHAPES = [(64, 256), (48, 256), (32, 256), (24, 256), (48, 192), (32, 192), (32, 128), (24, 128)]
+
+
+@pytest.mark.parametrize("kind,args", [("hex", (5,)), ("quad", (5,))])
+def test_patch_shape_and_autotuner_leave_results_unchanged(kind, args):
+    """Each edge's flux is evaluated from its half-edge of lower mesh index
+    with the same inputs whatever patch holds it, and the same code handles
+    every block size, so the state depends neither on the patch face cap nor
+    on the threads per workgroup: fixed shapes and the tuned one agree bit
+    for bit."""
     import splash_amd as S
-    m = S.icosphere_hex(5)
+    m = S.icosphere_hex(*args) if kind == "hex" else S.quad_sphere(*args)
     U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=4, omega=(0, 0, 0.3))
     out = []
-    for cap in (64, 32, 17, -1):
-        p = S.default_params(patch_faces=cap)
+    for cap, threads in [(64, 0), (32, 256), (17, 256), (48, 192), (40, 192), (32, 128), (20, 128), (-1, 0)]:
+        p = S.default_params(patch_faces=cap, block_threads=threads)
         s = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=p, precision=S.MIXED)
-        got = s.patch_faces()
-        assert (got == cap) if cap > 0 else got in (64, 48, 40, 32, 24)
+        faces, nt = s.patch_shape()
+        if cap > 0:
+            assert faces == cap and nt == (threads or 256)
+        else:
+            assert (faces, nt) in SHAPES
         s.do_steps(1e-4, 40)
         out.append(s.get_state())
-        print(f"patch_faces {cap} -> {got}")
+        print(f"patch_faces {cap}, block_threads {threads} -> {faces}, {nt}")
     for U in out[1:]:
         assert np.array_equal(U, out[0])
+
+
+def test_block_threads_rejects_other_sizes():
+    import splash_amd as S
+    m = S.icosphere_hex(2)
+    U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=4)
+    with pytest.raises(S.SplashError):
+        S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=S.default_params(block_threads=64), precision=S.MIXED)
 
 
 def test_ensemble_autotuned_equals_single_runs():
@@ -168,8 +185,8 @@ def test_ensemble_autotuned_equals_single_runs():
     tuned = S.default_params(patch_faces=-1)
     ens = S.Ensemble([(m, U0, g, om, p if p is not None else tuned) for m, U0, g, om, p, _ in runs], 5,
                      S.HLLCPLUS)
-    print("ensemble patch faces:", ens.patch_faces())
-    assert ens.patch_faces() in (64, 48, 40, 32, 24)
+    print("ensemble patch shape:", ens.patch_shape())
+    assert ens.patch_shape() in SHAPES
     dt0 = [r[5] for r in runs]
     ens.do_steps(dt0, 20)
     for r, (m, U0, g, om, p, d) in enumerate(runs):
