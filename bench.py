@@ -53,7 +53,10 @@ from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state,
 # sources whose hash it names (kernel_source_hash); for any other source tree
 # the bench line reports traffic as null.  The summary with the commands is
 # under profiles/.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    ("c3", "rotation03", "stage_1", "dc93b613313cc5d0"): 534_902_374,
+    ("c3", "rotation03", "stage_2", "dc93b613313cc5d0"): 598_514_380,
+}
 KERNEL_SOURCES = ["stage_mixed.hip", "stage_kernels.inc", "engine_types.hpp", "launch.hpp", "layout.cpp",
                   "run_setup.cpp"]
 
