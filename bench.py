@@ -14,7 +14,10 @@ Omega = 0.5, `--state rotation05`, the reference driver would stop on step
 icosphere_hex(7) HLLC+ at Omega = 0.5; c5a / c5b: uv_sphere(2048,1024) /
 icosphere_hex(9) HLLC+ with a spherical-harmonic density).
 
-Both precision paths run the same steps from the same state:
+Both precision paths run the same steps from the same state.  Each timed
+region (exactly --steps steps) follows --warmup untimed steps of the measured
+solver and about 40 ms of steps of a separate pacer solver with the same
+kernels, so the GPU starts it at the working clock of this kernel mix:
   value       = face-updates/s of the FP32-flux / FP64-state path,
   fp64        = face-updates/s of the plain FP64 path,
   l1_err      = relative L1 density error of the FP32-flux path vs FP64,
