@@ -162,7 +162,7 @@ void Engine::upload(const RunHost& r) {
         d.slot_b = static_cast<const float*>(dev(sizeof(float) * m.slot_b.size(), m.slot_b.data()));
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * m.edge32.size(), m.edge32.data()));
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * m.edge_rec.size(), m.edge_rec.data()));
-        d.acc_rate = static_cast<const double*>(dev(sizeof(double) * m.rate.size(), m.rate.data()));
+        d.acc_rate = static_cast<const float*>(dev(sizeof(float) * m.rate.size(), m.rate.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * m.inv_area.size(), m.inv_area.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_off.size(), L.fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_face.size(), L.fset_face.data()));

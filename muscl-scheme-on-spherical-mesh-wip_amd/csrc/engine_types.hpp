@@ -89,7 +89,7 @@ struct GeomPtrs {
     const uint32_t* edge_rec;
     const int32_t* fset_off;  // [n_blocks+1]
     const int32_t* fset_face; // internal face ids, each block's own faces first
-    const double* acc_rate;   // [nf] accretion rate profile (acc[.][0])
+    const float* acc_rate;    // [nf] accretion rate profile (acc[.][0]) in FP32, for the FP32 source
     const float* inv_area;    // [nf] 1 / area (FP32)
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)

@@ -90,8 +90,8 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     // the trailing padding blocks of the whole grid are empty
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
-    std::vector<float> slot_b, edge32, inv_area;
-    std::vector<double> fcn4, area, rate, U;
+    std::vector<float> slot_b, edge32, inv_area, rate;
+    std::vector<double> fcn4, area, U;
     std::vector<uint8_t> light_mask;
     std::vector<int2> seg;  // readout workgroups: each run's faces in blocks from its first face
     std::vector<int32_t> seg_run, run_part;
@@ -172,7 +172,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.slot_b = static_cast<const float*>(dev(sizeof(float) * slot_b.size(), slot_b.data()));
         d.edge32 = static_cast<const float*>(dev(sizeof(float) * edge32.size(), edge32.data()));
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * edge_rec.size(), edge_rec.data()));
-        d.acc_rate = static_cast<const double*>(dev(sizeof(double) * rate.size(), rate.data()));
+        d.acc_rate = static_cast<const float*>(dev(sizeof(float) * rate.size(), rate.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * inv_area.size(), inv_area.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));

@@ -284,7 +284,7 @@ MixedTables pack_mixed_tables(const RunHost& r) {
     m.rate.resize(nf);
     m.inv_area.resize(nf);
     for (int32_t k = 0; k < nf; ++k) {
-        m.rate[k] = r.acc[8 * size_t(k)];
+        m.rate[k] = float(r.acc[8 * size_t(k)]);
         m.inv_area[k] = float(1.0 / r.area[k]);
     }
     for (size_t q = 0; q + 1 < L.fset_off.size(); ++q) m.fset_max = std::max(m.fset_max, L.fset_off[q + 1] - L.fset_off[q]);

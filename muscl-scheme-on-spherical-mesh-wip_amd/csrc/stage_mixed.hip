@@ -484,7 +484,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     }
     // face data of phase 3, requested here so the flux phase hides the
     // loads without holding them through the reconstruction
-    double rate = 0;
+    float rate = 0.0f;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
     int hb = 0, he = 0;
     if (face) {
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
         for (int k = 0; k < DIM; ++k) src[k] = 0.0f;
         if constexpr (DIM == 5) {
             // on the face's stage input (p-form), from LDS
-            if (P.accretion_on) source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), float(rate), src);
+            if (P.accretion_on) source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), rate, src);
         }
         // Kahan-compensated FP64 sum of the FP32 edge terms: c carries the
         // (negated) low part each addition loses into the next term.  The
