@@ -88,12 +88,7 @@ PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0) {
     for (const PatchShape& c : kPatchShapeCandidates) {
         relayout(r, mixed::slots(c.threads), c.faces);
         Engine e(r, U_in, kMixed);
-        e.steps(dt0, 16);
-        e.synchronize();
-        const auto t0 = std::chrono::steady_clock::now();
-        e.steps(dt0, 32);
-        e.synchronize();
-        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double t = time_trial([&](int n) { e.steps(dt0, n); }, [&] { e.synchronize(); }, 32);
         if (best_t == 0 || t < best_t) {
             best_t = t;
             best = c;

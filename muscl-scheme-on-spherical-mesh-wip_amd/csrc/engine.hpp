@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -149,13 +150,32 @@ struct PatchShape {
 
 // The mixed-precision patch shape with the shortest measured step: each
 // candidate re-lays out r, runs a trial Engine on its own copy of the state
-// (16 graph-replayed warm-up steps, then 32 timed ones at dt0) and r is left
-// laid out for the winner, which is returned.
+// (time_trial: at least 20 ms of warm-up steps, then 32 timed ones at dt0)
+// and r is left laid out for the winner, which is returned.
 PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0);
 
 // faces per patch the mixed kernel allows, and the candidates of the tuner
 constexpr int32_t kPatchFacesMax = 64;
 constexpr PatchShape kPatchShapeCandidates[] = {{256, 64}, {256, 48}, {256, 32}, {256, 24},
                                                 {192, 48}, {192, 32}, {128, 32}, {128, 24}};
+
+// Seconds for `timed` steps of a tuner trial, after untimed groups of 16
+// steps (each followed by a wait) for at least warm_ms: after an idle gap
+// the GPU needs about 20 ms of load to reach its working clock, so a fixed
+// short warm-up would time the first candidates slow.  step(n) enqueues n
+// steps, sync() waits for them.
+template <class Step, class Sync>
+double time_trial(Step&& step, Sync&& sync, int timed, double warm_ms = 20.0) {
+    using clock = std::chrono::steady_clock;
+    const auto w0 = clock::now();
+    do {
+        step(16);
+        sync();
+    } while (std::chrono::duration<double, std::milli>(clock::now() - w0).count() < warm_ms);
+    const auto t0 = clock::now();
+    step(timed);
+    sync();
+    return std::chrono::duration<double>(clock::now() - t0).count();
+}
 
 }  // namespace splash

@@ -54,12 +54,8 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
         for (const PatchShape& c : kPatchShapeCandidates) {
             for (RunHost& h : hosts) relayout(h, mixed::slots(c.threads), c.faces);
             Ensemble trial(hosts, U_in, dim, solver);
-            trial.steps(dt0.data(), 16);
-            trial.synchronize();
-            const auto t0 = std::chrono::steady_clock::now();
-            trial.steps(dt0.data(), 32);
-            trial.synchronize();
-            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const double t =
+                time_trial([&](int n) { trial.steps(dt0.data(), n); }, [&] { trial.synchronize(); }, 32);
             if (best_t == 0 || t < best_t) {
                 best_t = t;
                 best = c;
