@@ -143,6 +143,7 @@ void Engine::upload(const RunHost& r) {
     d.slot_b = d.edge32 = nullptr;
     d.acc_rate = nullptr;
     d.inv_area = nullptr;
+    d.fc32 = nullptr;
     d.fset_off = d.fset_face = nullptr;
     d.fset_max = 0;
     d.slot_max = 0;
@@ -159,6 +160,7 @@ void Engine::upload(const RunHost& r) {
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * m.edge_rec.size(), m.edge_rec.data()));
         d.acc_rate = static_cast<const float*>(dev(sizeof(float) * m.rate.size(), m.rate.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * m.inv_area.size(), m.inv_area.data()));
+        d.fc32 = static_cast<const float*>(dev(sizeof(float) * m.fc32.size(), m.fc32.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_off.size(), L.fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * L.fset_face.size(), L.fset_face.data()));
         d.fset_max = m.fset_max;

@@ -44,7 +44,7 @@ struct PhysParams {
     // FP32 copies of what the mixed kernel's FP32 arithmetic uses, so its
     // waves neither divide nor convert the uniform FP64 parameters
     struct {
-        float gam, inv_gm1, kq;  // gam, 1/(gam-1), (gam+1)/(2 gam)
+        float gam, inv_gm1, kq, gm1;  // gam, 1/(gam-1), (gam+1)/(2 gam), gam-1
         float gam_h, gam3d_rho, cool_k, e_acc, alpha, omega_ns, dmdt_coef;
         float om_acc[3];
     } f;
@@ -91,6 +91,7 @@ struct GeomPtrs {
     const int32_t* fset_face; // internal face ids, each block's own faces first
     const float* acc_rate;    // [nf] accretion rate profile (acc[.][0]) in FP32, for the FP32 source
     const float* inv_area;    // [nf] 1 / area (FP32)
+    const float* fc32;        // [nf][4] unit face centre in FP32, accretion rate (the FP32 stage-1 epilogue)
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)

@@ -86,7 +86,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     // the trailing padding blocks of the whole grid are empty
     std::vector<int32_t> foff{0}, blk_face, halo_off, fset_off, fset_face, blk_run, run_blk;
     std::vector<uint32_t> slot_a, edge_rec;
-    std::vector<float> slot_b, edge32, inv_area, rate;
+    std::vector<float> slot_b, edge32, inv_area, rate, fc32;
     std::vector<double> fcn4, area, U;
     std::vector<uint8_t> light_mask;
     std::vector<int2> seg;  // readout workgroups: each run's faces in blocks from its first face
@@ -118,6 +118,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         append(edge_rec, m.edge_rec);
         append(rate, m.rate);
         append(inv_area, m.inv_area);
+        append(fc32, m.fc32);
         append(fcn4, rh.fcn4);
         append(area, rh.area);
         if (dim == 5) append(light_mask, rh.light_mask);
@@ -170,6 +171,7 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.edge_rec = static_cast<const uint32_t*>(dev(sizeof(uint32_t) * edge_rec.size(), edge_rec.data()));
         d.acc_rate = static_cast<const float*>(dev(sizeof(float) * rate.size(), rate.data()));
         d.inv_area = static_cast<const float*>(dev(sizeof(float) * inv_area.size(), inv_area.data()));
+        d.fc32 = static_cast<const float*>(dev(sizeof(float) * fc32.size(), fc32.data()));
         d.fset_off = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_off.size(), fset_off.data()));
         d.fset_face = static_cast<const int32_t*>(dev(sizeof(int32_t) * fset_face.size(), fset_face.data()));
         d.fset_max = fset_max;

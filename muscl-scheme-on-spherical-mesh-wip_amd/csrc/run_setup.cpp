@@ -83,6 +83,7 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
     P.f.gam = float(gam);
     P.f.inv_gm1 = float(1 / (gam - 1));
     P.f.kq = float((gam + 1) / (2 * gam));
+    P.f.gm1 = float(gam - 1);
     P.f.gam_h = float(P.gam_h);
     P.f.gam3d_rho = float(P.gam3d_rho);
     P.f.cool_k = float(P.cool_k);
@@ -283,9 +284,12 @@ MixedTables pack_mixed_tables(const RunHost& r) {
     }
     m.rate.resize(nf);
     m.inv_area.resize(nf);
+    m.fc32.resize(4 * size_t(nf));
     for (int32_t k = 0; k < nf; ++k) {
         m.rate[k] = float(r.acc[8 * size_t(k)]);
         m.inv_area[k] = float(1.0 / r.area[k]);
+        for (int c = 0; c < 3; ++c) m.fc32[4 * size_t(k) + c] = float(r.fcn4[4 * size_t(k) + c]);
+        m.fc32[4 * size_t(k) + 3] = m.rate[k];
     }
     for (size_t q = 0; q + 1 < L.fset_off.size(); ++q) m.fset_max = std::max(m.fset_max, L.fset_off[q + 1] - L.fset_off[q]);
     for (int32_t b = 0; b < nb; ++b) {

@@ -70,9 +70,11 @@ void light_observers(double obs[16]);
 //   edge_rec [n_ext/2]: evaluating slot | twin slot << 9 | lower mesh index << 18
 //   rate [nf]: accretion rate profile, rounded to FP32 (the FP32 source is its only user)
 //   inv_area [nf]: 1 / area (the limiter's Courant number is arc * dt / area)
+//   fc32 [nf][4]: unit face centre rounded to FP32, the rate above (stage 1,
+//                 whose result only feeds the FP32 input of stage 2)
 struct MixedTables {
     std::vector<uint32_t> slot_a, edge_rec;
-    std::vector<float> slot_b, edge32, inv_area;
+    std::vector<float> slot_b, edge32, inv_area, fc32;
     std::vector<float> rate;
     int32_t fset_max = 1;
     int32_t slot_max = 4;  // largest own + halo slot count of a patch, a multiple of 4

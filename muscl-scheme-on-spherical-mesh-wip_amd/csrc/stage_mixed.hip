@@ -483,16 +483,27 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
         if (t1 < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r1, &sUP[t1 * DIM]);
     }
     // face data of phase 3, requested here so the flux phase hides the
-    // loads without holding them through the reconstruction
+    // loads without holding them through the reconstruction.  Stage 1's
+    // result only feeds the FP32 input of stage 2, so it takes the face
+    // centre and the accretion rate from one FP32 row; stage 2 updates the
+    // FP64 state and reads the FP64 centre and round_diff / area.
     float rate = 0.0f;
     V3<double> fcn = v3(0.0, 0.0, 0.0);
+    double rd_area = 0.0;
     int hb = 0, he = 0;
     if (face) {
         hb = *row(g.foff, uint32_t(i));
         he = *row(g.foff, uint32_t(i + 1));
-        const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
-        fcn = v3(c4.x, c4.y, c4.z);
-        if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
+        if constexpr (STAGE == 1) {
+            const float4 c4 = *row(reinterpret_cast<const float4*>(g.fc32), uint32_t(i));
+            fcn = v3(double(c4.x), double(c4.y), double(c4.z));
+            rate = c4.w;
+        } else {
+            const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));
+            fcn = v3(c4.x, c4.y, c4.z);
+            rd_area = c4.w;
+            if (DIM == 5 && P.accretion_on) rate = *row(g.acc_rate, uint32_t(i));
+        }
     }
     uint32_t rec = 0;
     float4 e4, n4;
@@ -543,14 +554,19 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     // combine and density floor for one face per thread
     double vmax = 1e-8, dtx = 1e20;
     if (face) {
-        // U0 and 1/A are first needed after the source and the edge sum,
-        // which hide these loads
-        double u0[DIM];
-        const double* ui = row(U, uint32_t(i * DIM));
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
-        const double rd_area = *row(g.fcn, uint32_t(4 * i + 3));  // round_diff / area
+        double u0[DIM];  // stage 2: the FP64 state
+        double coef;     // round_diff / area * dt_here
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
+        if constexpr (STAGE == 1) {
+            coef = 0.5 * P.round_diff * double(sA[t]);  // sA = dt / area in FP32
+        } else {
+            // U0 is first needed after the source and the edge sum, which
+            // hide this load
+            const double* ui = row(U, uint32_t(i * DIM));
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
+            coef = rd_area * dt_here;
+        }
         float src[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) src[k] = 0.0f;
@@ -578,15 +594,34 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             }
         }
         if (nan_flux) raise_stop(sc, splash::kStopNaNFlux, i);
-        const double coef = rd_area * dt_here;
-        double un[DIM];
+        if constexpr (STAGE == 1) {
+            // stage 1's result only feeds the FP32 input of stage 2: the
+            // increments leave the compensated FP64 sums rounded to FP32 and
+            // are added to the face's own FP32 stage input (LDS entry t, the
+            // p-form of U0); the pressure moves by (gam-1) (dE - dK), K =
+            // |r x L|^2 / (2 Sigma), so no E-form state is formed
+            float inc[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[k] = (dt_here * double(src[k]) - coef * (sum[k] - c[k])) + u0[k];
-        if (un[0] < 1e-8) un[0] = 1e-8;
-        double wn[DIM];
-        face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
-        store_w32<DIM>(W32_out, g.nf, i, wn);
-        if constexpr (STAGE == 2) {
+            for (int k = 0; k < DIM; ++k) inc[k] = float(dt_here * double(src[k]) - coef * (sum[k] - c[k]));
+            const float4 x = sW[t];
+            float s1 = x.x + inc[0];
+            if (s1 < 1e-8f) s1 = 1e-8f;
+            const float4 w1 = make_float4(s1, x.y + inc[1], x.z + inc[2], x.w + inc[3]);
+            *row(reinterpret_cast<float4*>(W32_out), uint32_t(i)) = w1;
+            if constexpr (DIM == 5) {
+                const V3<float> cf = v3(float(fcn.x), float(fcn.y), float(fcn.z));  // the FP32 centre, exact
+                const V3<float> r0 = cross(cf, v3(x.y, x.z, x.w)), r1 = cross(cf, v3(w1.y, w1.z, w1.w));
+                const float dK = 0.5f * (dot(r1, r1) * __builtin_amdgcn_rcpf(w1.x) - dot(r0, r0) * __builtin_amdgcn_rcpf(x.x));
+                *row(W32_out, uint32_t(4 * g.nf + i)) = smax(1e-16f, fmaf(P.f.gm1, inc[4] - dK, sP[t]));
+            }
+        } else {
+            double un[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) un[k] = (dt_here * double(src[k]) - coef * (sum[k] - c[k])) + u0[k];
+            if (un[0] < 1e-8) un[0] = 1e-8;
+            double wn[DIM];
+            face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
+            store_w32<DIM>(W32_out, g.nf, i, wn);
 #pragma unroll
             for (int k = 0; k < DIM; ++k) row(U, uint32_t(i * DIM))[k] = un[k];
         }
