@@ -78,7 +78,9 @@ typedef struct splash_params {
     int32_t precision; /* SPLASH_FP64 or SPLASH_MIXED */
     int32_t setup_threads; /* host threads for the geometry precompute; 0 = every host thread
                             * (OpenMP's default team); the tables do not depend on it */
-    int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..64; 0 = 64;
+    int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..128 (above 64 two
+                          * waves update a patch's faces); 0 = 112 for 256-thread workgroups on
+                          * meshes of mean valence at most 4.5, else 64 (per run in an ensemble);
                           * -1 = the fastest of a few (block_threads, patch_faces) candidates,
                           * measured at creation */
     int32_t no_graph;    /* 0: splash_do_steps replays a captured HIP graph (captured after the
@@ -253,10 +255,11 @@ int splash_ensemble_synchronize(splash_ensemble *e);
 int splash_ensemble_info(const splash_ensemble *e, int32_t *n_runs, int64_t *total_faces, int64_t *total_halfedges,
                          int32_t *n_blocks);
 int splash_ensemble_num_faces(const splash_ensemble *e, int run, int32_t *n_faces);
-/* faces per patch in use (the tuned value when params[0].patch_faces = -1) */
+/* faces per patch in use (the tuned value when params[0].patch_faces = -1;
+ * run 0's default cap when it is 0) */
 int splash_ensemble_patch_faces(const splash_ensemble *e, int32_t *patch_faces);
 /* faces per patch and threads per workgroup in use (params[0] sets both, or
- * the tuner when params[0].patch_faces = -1) */
+ * the tuner when params[0].patch_faces = -1; run 0's default cap when it is 0) */
 int splash_ensemble_patch_shape(const splash_ensemble *e, int32_t *patch_faces, int32_t *block_threads);
 int splash_ensemble_time(splash_ensemble *e, int run, double *t);
 int splash_ensemble_last_dt(splash_ensemble *e, int run, double *dt);
@@ -275,11 +278,16 @@ int splash_ensemble_diagnostics(splash_ensemble *e, int run, double out[5]);
 int splash_ensemble_light_curves(splash_ensemble *e, int run, double out[4]);
 
 /* Host-side check of the internal GPU layout of a mesh that the
- * mixed-precision kernel uses (coordinate-bisection face order, patches of at most 64
- * faces whose half-edges plus halo fit 512 LDS slots, LDS face sets):
+ * mixed-precision kernel uses (coordinate-bisection face order, here patches of at most
+ * 64 faces whose half-edges plus halo fit 512 LDS slots, LDS face sets):
  * builds and validates it and returns {blocks (multiple of 8), non-empty
  * blocks, halo half-edges, max halo, max face set}. */
 int splash_layout_stats(const splash_mesh_desc *mesh, int32_t out[5]);
+/* The same for a given stage-kernel shape: block_threads 128, 192 or 256
+ * (0 = 256), patch_faces 1..128 or 0 for the default cap of that mesh and
+ * block size (splash_params.patch_faces). */
+int splash_layout_stats_shape(const splash_mesh_desc *mesh, int32_t block_threads, int32_t patch_faces,
+                              int32_t out[5]);
 
 /* Human-readable text of the last error on this host thread. */
 const char *splash_last_error_message(void);

@@ -359,12 +359,18 @@ int splash_geometry_get_view(const splash_geometry* g, splash_geometry_view* v) 
 
 void splash_geometry_free(splash_geometry* g) { delete g; }
 
-int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
+namespace {
+
+int layout_stats(const splash_mesh_desc* mesh, int32_t threads, int32_t faces, int32_t out[5]) {
     if (!out) return fail(SPLASH_EINVAL, "null output");
+    if (threads == 0) threads = splash::mixed::default_threads();
+    if (!splash::mixed::valid_threads(threads)) return fail(SPLASH_EINVAL, "block_threads must be 0, 128, 192 or 256");
+    if (faces < 0 || faces > splash::kPatchFacesMax) return fail(SPLASH_EINVAL, "patch_faces out of range");
     return guarded([&] {
-        const splash::Geometry g = splash::build_geometry(from_desc(mesh), 0);
-        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(splash::mixed::default_threads()),
-                                                     splash::mixed::max_faces());
+        const splash::PolyMesh m = from_desc(mesh);
+        const splash::Geometry g = splash::build_geometry(m, 0);
+        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(threads),
+                                                     faces ? faces : splash::default_patch_faces(m, threads));
         splash::validate_layout(g, L);
         int32_t max_halo = 0, nonempty = 0;
         for (int32_t b = 0; b < L.n_blocks(); ++b) {
@@ -380,6 +386,17 @@ int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
         out[4] = max_fset;
         return SPLASH_OK;
     });
+}
+
+}  // namespace
+
+int splash_layout_stats(const splash_mesh_desc* mesh, int32_t out[5]) {
+    return layout_stats(mesh, 0, splash::kPatchFacesOneWave, out);
+}
+
+int splash_layout_stats_shape(const splash_mesh_desc* mesh, int32_t block_threads, int32_t patch_faces,
+                              int32_t out[5]) {
+    return layout_stats(mesh, block_threads, patch_faces, out);
 }
 
 int splash_ensemble_create(int n_runs, const splash_mesh_desc* meshes, const double* const* U_in, int dim,

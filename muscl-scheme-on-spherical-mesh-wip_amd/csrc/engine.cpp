@@ -28,8 +28,8 @@ Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, do
     if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
     if (block_threads != 0 && !mixed::valid_threads(block_threads))
         throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
-    const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
     const int32_t threads = block_threads ? block_threads : mixed::default_threads();
+    const int32_t cap = patch_faces > 0 ? patch_faces : default_patch_faces(mesh, threads);
     RunHost r = precision == kMixed
                     ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(threads), cap,
                                      geometry_threads)
@@ -51,6 +51,8 @@ void Engine::init(const RunHost& r, const double* U_in) {
     area_coeff_ = r.area_coeff;
     total_mass_ = r.total_mass;
     patch_faces_ = r.L.max_faces;
+    if (precision_ == kMixed && patch_faces_ > mixed::max_faces())
+        throw std::logic_error("patch face cap exceeds the faces the stage kernel updates per patch");
     block_threads_ = precision_ == kMixed ? r.L.slots / 2 : fp64::block_size();
     nf_ = r.geo.nf;
     nh_ = r.geo.nh;
@@ -83,7 +85,7 @@ void Engine::init(const RunHost& r, const double* U_in) {
 }
 
 PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0) {
-    PatchShape best{mixed::default_threads(), kPatchFacesMax};
+    PatchShape best{mixed::default_threads(), kPatchFacesOneWave};
     double best_t = 0;
     for (const PatchShape& c : kPatchShapeCandidates) {
         relayout(r, mixed::slots(c.threads), c.faces);

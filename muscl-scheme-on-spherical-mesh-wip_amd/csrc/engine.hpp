@@ -154,10 +154,21 @@ struct PatchShape {
 // and r is left laid out for the winner, which is returned.
 PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0);
 
-// faces per patch the mixed kernel allows, and the candidates of the tuner
-constexpr int32_t kPatchFacesMax = 64;
-constexpr PatchShape kPatchShapeCandidates[] = {{256, 64}, {256, 48}, {256, 32}, {256, 24},
-                                                {192, 48}, {192, 32}, {128, 32}, {128, 24}};
+// faces per patch the mixed kernel allows (two face waves of 64), and the
+// candidates of the tuner
+constexpr int32_t kPatchFacesMax = 128;
+constexpr PatchShape kPatchShapeCandidates[] = {{256, 112}, {256, 96}, {256, 64}, {256, 48},
+                                                {256, 32}, {192, 48}, {192, 32}, {128, 32}};
+
+// The patch face cap when none is set: 112 (two face waves) for 256-thread
+// workgroups on meshes of mean valence at most 4.5, whose 512 slots then
+// hold about 110 quads; 64 (one face wave) otherwise, since hexagons fill
+// the slots at about 70 faces and a second wave would update a handful.
+inline int32_t default_patch_faces(const PolyMesh& m, int32_t threads) {
+    const bool low_valence = 2 * int64_t(m.n_halfedges()) <= 9 * int64_t(m.n_faces());
+    return threads == 256 && low_valence ? 112 : 64;
+}
+constexpr int32_t kPatchFacesOneWave = 64;
 
 // Seconds for `timed` steps of a tuner trial, after untimed groups of 16
 // steps (each followed by a wait) for at least warm_ms: after an idle gap

@@ -35,12 +35,13 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
     if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
     if (block_threads != 0 && !mixed::valid_threads(block_threads))
         throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
-    const int32_t cap = patch_faces > 0 ? patch_faces : kPatchFacesMax;
     const int32_t threads = block_threads ? block_threads : mixed::default_threads();
     std::vector<RunHost> hosts;
     std::vector<const double*> U_in;
     for (const Run& run : runs) {
         if (!run.mesh) throw std::invalid_argument("null mesh");
+        // without a set cap each run takes its own mesh's default
+        const int32_t cap = patch_faces > 0 ? patch_faces : default_patch_faces(*run.mesh, threads);
         hosts.push_back(build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
                                        mixed::slots(threads), cap, geometry_threads));
         U_in.push_back(run.U_in);
@@ -48,7 +49,7 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
     if (patch_faces == -1) {
         // the fastest patch shape for the whole ensemble, measured on trial
         // ensembles with their own copies of the states
-        PatchShape best{threads, kPatchFacesMax};
+        PatchShape best{threads, kPatchFacesOneWave};
         double best_t = 0;
         std::vector<double> dt0(runs.size(), 1e-4);
         for (const PatchShape& c : kPatchShapeCandidates) {
@@ -78,8 +79,11 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
     const int R = int(hosts.size());
     patch_faces_ = hosts[0].L.max_faces;
     block_threads_ = hosts[0].L.slots / 2;
-    for (const RunHost& h : hosts)
+    for (const RunHost& h : hosts) {
         if (h.L.slots != hosts[0].L.slots) throw std::invalid_argument("ensemble runs laid out for different blocks");
+        if (h.L.max_faces > mixed::max_faces())
+            throw std::logic_error("patch face cap exceeds the faces the stage kernel updates per patch");
+    }
 
     // merged tables: every run's arrays appended, indices shifted by the
     // faces / half-edges / halo slots / face-set entries before it; only

@@ -34,11 +34,11 @@ __device__ __forceinline__ float m_pow(float x, float y) {
 // accretion / cooling / friction sources run in FP32 on the FP32 stage
 // input.
 
-// Patches: at most kFaces faces (one wave updates them) whose own
-// half-edges plus halo fit 2 * NT LDS slots, two slots per thread of an
+// Patches: at most kFaces faces (waves 0 and 1 update them, 64 each) whose
+// own half-edges plus halo fit 2 * NT LDS slots, two slots per thread of an
 // NT-thread workgroup.  NT is one of the block shapes below, chosen per run
 // (or ensemble) by the caller or the tuner.
-constexpr int kFaces = 64;
+constexpr int kFaces = 128;
 constexpr int kThreadChoices[] = {128, 192, 256};
 static_assert(2 * 256 <= 512, "slot indices are packed in 9 bits");
 
@@ -52,6 +52,40 @@ __device__ __forceinline__ T* row(T* base, uint32_t idx) {
 template <class T>
 __device__ __forceinline__ const T* row(const T* base, uint32_t idx) {
     return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * uint32_t(sizeof(T)));
+}
+
+// The dt limits of a patch's two face waves, combined in LDS without a
+// barrier: each wave folds its max(|v|, c_s) and min friction dt into the
+// block's LDS words with workgroup-scope atomics, and the second wave to
+// arrive writes the block's partial.  vmax > 0 orders like its bit pattern;
+// dtx (any sign, never NaN: the comparisons skip NaN candidates) is mapped to
+// an unsigned key of the same order.
+struct DtCombine {
+    unsigned long long v, x;
+    unsigned arrived;
+};
+__device__ __forceinline__ unsigned long long order_key(double d) {
+    const long long b = __double_as_longlong(d);
+    return b >= 0 ? (unsigned long long)(b) | 0x8000000000000000ull : ~(unsigned long long)(b);
+}
+__device__ __forceinline__ double from_order_key(unsigned long long k) {
+    return __longlong_as_double((long long)((k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+__device__ __forceinline__ void dt_combine_init(DtCombine* c) {
+    c->v = (unsigned long long)(__double_as_longlong(1e-8));
+    c->x = order_key(1e20);
+    c->arrived = 0;
+}
+// lane 0 of each of the two face waves, once per block
+__device__ __forceinline__ void dt_combine(DtCombine* c, double* partial, int b, double vmax, double dtx) {
+    __hip_atomic_fetch_max(&c->v, (unsigned long long)(__double_as_longlong(vmax)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_min(&c->x, order_key(dtx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (__hip_atomic_fetch_add(&c->arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1u) {
+        partial[2 * b] = __longlong_as_double(
+            (long long)(__hip_atomic_load(&c->v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)));
+        partial[2 * b + 1] = from_order_key(__hip_atomic_load(&c->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    }
 }
 
 template <int DIM>
@@ -435,7 +469,10 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     const uint32_t ext0 = uint32_t(hbase + hoff), edge_base = ext0 >> 1;
 
     const int i = f0 + t;
-    const bool face = i < f1;  // t < kFaces
+    const bool face = i < f1;  // t < kFaces: waves 0 and 1
+
+    __shared__ DtCombine s_dt;
+    if (STAGE == 2 && t == 0) dt_combine_init(&s_dt);  // before the first barrier
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
     // faces first, so face f0 + t sits at local index t); the first slot's
@@ -627,11 +664,12 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
         }
     }
     if constexpr (STAGE == 2) {
-        // this block's partial of max(|v|, c_s) and min friction dt: its faces
-        // are all in wave 0 (kFaces = 64), so the reduction stays in that
-        // wave and the other waves are already done
-        static_assert(kFaces == 64, "one wave holds a patch's faces");
-        if (t < 64) {
+        // this block's partial of max(|v|, c_s) and min friction dt: its
+        // faces are in waves 0 and 1 (kFaces = 128; wave 1 holds neutral
+        // values when it has none), each reduces its own, and the two are
+        // combined in LDS (dt_combine), so neither waits for the other
+        static_assert(kFaces == 128, "waves 0 and 1 hold a patch's faces");
+        if (t < 128) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const double ov = __shfl_xor(vmax, o);
@@ -639,13 +677,10 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
                 if (ov > vmax) vmax = ov;
                 if (dtx > ox) dtx = ox;
             }
-            if (t == 0) {
-                g.dt_partial[2 * b] = vmax;
-                g.dt_partial[2 * b + 1] = dtx;
-                if (b == (g.run_blk ? g.run_blk[run] : 0)) {  // the run's first patch advances its clock
-                    sc->t += sc->dt;
-                    sc->steps += 1;
-                }
+            if ((t & 63) == 0) dt_combine(&s_dt, g.dt_partial, b, vmax, dtx);
+            if (t == 0 && b == (g.run_blk ? g.run_blk[run] : 0)) {  // the run's first patch advances its clock
+                sc->t += sc->dt;
+                sc->steps += 1;
             }
         }
     }
@@ -687,7 +722,10 @@ __global__ __launch_bounds__(kBlock) void k_prepare32(GeomPtrs g, const PhysPara
     const int b = xcd_block();
     const PhysParams P = Pv[g.blk_run ? g.blk_run[b] : 0];
     const int t = threadIdx.x;
-    if (t >= 64) return;  // a patch's faces fit one wave
+    __shared__ DtCombine s_dt;
+    if (t == 0) dt_combine_init(&s_dt);
+    __syncthreads();
+    if (t >= kFaces) return;  // a patch's faces fit waves 0 and 1
     const int i = g.blk_face[b] + t;
     double vmax = 1e-8, dtx = 1e20;
     if (i < g.blk_face[b + 1]) {
@@ -706,10 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare32(GeomPtrs g, const PhysPara
         if (ov > vmax) vmax = ov;
         if (dtx > ox) dtx = ox;
     }
-    if (t == 0) {
-        g.dt_partial[2 * b] = vmax;
-        g.dt_partial[2 * b + 1] = dtx;
-    }
+    if ((t & 63) == 0) dt_combine(&s_dt, g.dt_partial, b, vmax, dtx);
 }
 
 void enqueue_prepare(const StepArgs& a, hipStream_t s) {

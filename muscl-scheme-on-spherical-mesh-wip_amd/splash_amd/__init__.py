@@ -125,12 +125,17 @@ class Geometry:
             lib().splash_geometry_free(h)
 
 
-def layout_stats(mesh):
+def layout_stats(mesh, block_threads=None, patch_faces=None):
     """(blocks, non-empty blocks, halo half-edges, max halo, max LDS face set)
-    of the validated internal GPU layout of `mesh` (host only)."""
+    of the validated internal GPU layout of `mesh` (host only): 64-face
+    patches in 256-thread workgroups, or the given shape (patch_faces 0: the
+    mesh's default cap)."""
     out = (C.c_int32 * 5)()
     d = mesh.desc()
-    check(lib().splash_layout_stats(C.byref(d), out))
+    if block_threads is None and patch_faces is None:
+        check(lib().splash_layout_stats(C.byref(d), out))
+    else:
+        check(lib().splash_layout_stats_shape(C.byref(d), int(block_threads or 0), int(patch_faces or 0), out))
     return tuple(out)
 
 

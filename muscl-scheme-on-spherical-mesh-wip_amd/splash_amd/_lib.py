@@ -80,6 +80,7 @@ SIGNATURES = {
     "splash_geometry_get_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
     "splash_geometry_free": (None, [C.c_void_p]),
     "splash_layout_stats": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_int32)]),
+    "splash_layout_stats_shape": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "splash_ensemble_create": (C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(C.POINTER(C.c_double)), C.c_int,
                                          C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),
                                          C.POINTER(C.c_void_p)]),

@@ -8,6 +8,7 @@ import shutil
 import subprocess
 
 import numpy as np
+import pytest
 
 import oracle_lib as O
 
@@ -109,3 +110,33 @@ def test_gpu_layout_validates_on_every_mesh_family():
         assert 0 < max_fset <= 1024
         assert nonempty == -(-mesh.n_faces // 64)
         assert halo < 0.19 * mesh.n_halfedges
+
+
+def test_gpu_layout_validates_at_every_patch_shape():
+    """Patches of up to 128 faces (two face waves) in 128-, 192- and
+    256-thread workgroups pass the same host-side index checks on every
+    generator; the default cap gives quad and triangle meshes two-wave
+    patches in 256-thread workgroups (the 512 slots hold about 110 quads,
+    fewer patches and a shorter halo than 64-face patches) and hexagon
+    meshes one-wave patches, whose 512 slots fill at about 70 faces."""
+    import splash_amd as S
+    meshes = (S.icosphere_hex(5), S.quad_sphere(5), S.uv_sphere(96, 48), S.icosphere(4), S.icosphere_hex(0))
+    for mesh in meshes:
+        for threads in (128, 192, 256):
+            for faces in (0, 24, 64, 96, 112, 128):
+                blocks, nonempty, halo, max_halo, max_fset = S.layout_stats(mesh, threads, faces)
+                assert blocks % 8 == 0 and 0 < nonempty <= blocks
+                assert 0 <= halo < mesh.n_halfedges and max_halo < 2 * threads
+                assert 0 < max_fset <= 1024
+                if faces:
+                    assert nonempty >= -(-mesh.n_faces // faces)
+    hexm, quad = meshes[0], meshes[1]
+    assert S.layout_stats(hexm, 256, 0) == S.layout_stats(hexm, 256, 64)
+    assert S.layout_stats(quad, 256, 0) == S.layout_stats(quad, 256, 112)
+    assert S.layout_stats(quad, 192, 0) == S.layout_stats(quad, 192, 64)
+    one_wave, two_wave = S.layout_stats(quad, 256, 64), S.layout_stats(quad, 256, 112)
+    assert two_wave[1] < 0.65 * one_wave[1]   # patches
+    assert two_wave[2] < 0.85 * one_wave[2]   # halo half-edges
+    for bad in ((64, 64), (256, 129), (256, -1)):
+        with pytest.raises(S.SplashError):
+            S.layout_stats(quad, *bad)

@@ -142,7 +142,7 @@ def test_ensemble_rejects_mixed_solver_shapes():
         S.Ensemble([(m, U4, GAM, 0.0, None)], 4, S.HLLCPLUS)  # dim 4 with an adiabatic solver
 
 
-SHAPES = [(64, 256), (48, 256), (32, 256), (24, 256), (48, 192), (32, 192), (32, 128), (24, 128)]
+SHAPES = [(112, 256), (96, 256), (64, 256), (48, 256), (32, 256), (48, 192), (32, 192), (32, 128)]
 
 
 @pytest.mark.parametrize("kind,args", [("hex", (5,)), ("quad", (5,))])
@@ -150,18 +150,22 @@ def test_patch_shape_and_autotuner_leave_results_unchanged(kind, args):
     """Each edge's flux is evaluated from its half-edge of lower mesh index
     with the same inputs whatever patch holds it, and the same code handles
     every block size, so the state depends neither on the patch face cap nor
-    on the threads per workgroup: fixed shapes and the tuned one agree bit
-    for bit."""
+    on the threads per workgroup: fixed shapes (one and two face waves per
+    patch), the default one and the tuned one agree bit for bit."""
     import splash_amd as S
     m = S.icosphere_hex(*args) if kind == "hex" else S.quad_sphere(*args)
     U0 = S.ic.adiabatic_state(m.face_centres(), GAM, seed=4, omega=(0, 0, 0.3))
     out = []
-    for cap, threads in [(64, 0), (32, 256), (17, 256), (48, 192), (40, 192), (32, 128), (20, 128), (-1, 0)]:
+    shapes = [(0, 0), (64, 0), (112, 256), (128, 256), (32, 256), (17, 256), (48, 192), (100, 192), (40, 192),
+              (32, 128), (20, 128), (-1, 0)]
+    for cap, threads in shapes:
         p = S.default_params(patch_faces=cap, block_threads=threads)
         s = S.MUSCL_HLLCplus(m, U0, 5, GAM, 0.0, params=p, precision=S.MIXED)
         faces, nt = s.patch_shape()
         if cap > 0:
             assert faces == cap and nt == (threads or 256)
+        elif cap == 0:
+            assert (faces, nt) == ((112 if kind == "quad" else 64), 256)
         else:
             assert (faces, nt) in SHAPES
         s.do_steps(1e-4, 40)
