@@ -367,6 +367,7 @@ def main():
     stop = smx.get_stop_check()
     tmx = smx.time()
     dt_eff = smx.last_dt()
+    patch_faces, block_threads = smx.patch_shape()
     prof = profile(smx, args.dt, args.profile_steps)
     loop_ms, step_ms = driver_loop(smx, args.dt, args.driver_steps)
     del smx
@@ -407,6 +408,7 @@ def main():
             "workload": f"{args.workload}: {mesh_name}{margs} F={F} H={H}, dim 5 MUSCL_{solver}, gam3d 4/3, "
                         f"omega_ns={omega_ns}, {state}, {sources}, dt0={args.dt}",
             "faces": F, "halfedges": H, "dt0": args.dt, "dt_effective": dt_eff,
+            "patch_faces": patch_faces, "block_threads": block_threads,
         },
         "l1_err_vs_fp64": l1,
         "linf_err_vs_fp64": linf,
