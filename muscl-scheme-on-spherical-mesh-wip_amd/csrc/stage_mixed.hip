@@ -591,7 +591,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     // combine and density floor for one face per thread
     double vmax = 1e-8, dtx = 1e20;
     if (face) {
-        double u0[DIM];  // stage 2: the FP64 state
+        double u0[DIM];  // the FP64 state (stage 2, and stage 1's faces formed from it)
         double coef;     // round_diff / area * dt_here
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
         if constexpr (STAGE == 1) {
@@ -631,12 +631,18 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             }
         }
         if (nan_flux) raise_stop(sc, splash::kStopNaNFlux, i);
+        // stage 1's result only feeds the FP32 input of stage 2: the
+        // increments leave the compensated FP64 sums rounded to FP32 and are
+        // added to the face's own FP32 stage input (LDS entry t, the p-form
+        // of U0); the pressure moves by (gam-1) (dE - dK), K = |r x L|^2 /
+        // (2 Sigma), so no E-form state is formed.  That difference cancels
+        // where the kinetic energy dwarfs the pressure (near vacuum, a
+        // floored density): a face whose K exceeds 8 times its pressure
+        // before or after (or with any non-finite value) is formed from the
+        // FP64 state instead, as stage 2 forms its faces, so the FP32
+        // rounding stays within a few ulp of the pressure.
+        bool fast = STAGE == 1;
         if constexpr (STAGE == 1) {
-            // stage 1's result only feeds the FP32 input of stage 2: the
-            // increments leave the compensated FP64 sums rounded to FP32 and
-            // are added to the face's own FP32 stage input (LDS entry t, the
-            // p-form of U0); the pressure moves by (gam-1) (dE - dK), K =
-            // |r x L|^2 / (2 Sigma), so no E-form state is formed
             float inc[DIM];
 #pragma unroll
             for (int k = 0; k < DIM; ++k) inc[k] = float(dt_here * double(src[k]) - coef * (sum[k] - c[k]));
@@ -644,14 +650,27 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             float s1 = x.x + inc[0];
             if (s1 < 1e-8f) s1 = 1e-8f;
             const float4 w1 = make_float4(s1, x.y + inc[1], x.z + inc[2], x.w + inc[3]);
-            *row(reinterpret_cast<float4*>(W32_out), uint32_t(i)) = w1;
+            float p1 = 0.0f;
             if constexpr (DIM == 5) {
                 const V3<float> cf = v3(float(fcn.x), float(fcn.y), float(fcn.z));  // the FP32 centre, exact
                 const V3<float> r0 = cross(cf, v3(x.y, x.z, x.w)), r1 = cross(cf, v3(w1.y, w1.z, w1.w));
-                const float dK = 0.5f * (dot(r1, r1) * __builtin_amdgcn_rcpf(w1.x) - dot(r0, r0) * __builtin_amdgcn_rcpf(x.x));
-                *row(W32_out, uint32_t(4 * g.nf + i)) = smax(1e-16f, fmaf(P.f.gm1, inc[4] - dK, sP[t]));
+                const float k0 = 0.5f * dot(r0, r0) * __builtin_amdgcn_rcpf(x.x);
+                const float k1 = 0.5f * dot(r1, r1) * __builtin_amdgcn_rcpf(w1.x);
+                p1 = smax(1e-16f, fmaf(P.f.gm1, inc[4] - (k1 - k0), sP[t]));
+                fast = smax(k0, k1) <= 8.0f * smin(sP[t], p1);  // false for NaN
             }
-        } else {
+            if (fast) {
+                *row(reinterpret_cast<float4*>(W32_out), uint32_t(i)) = w1;
+                if constexpr (DIM == 5) *row(W32_out, uint32_t(4 * g.nf + i)) = p1;
+            } else {
+                const double* ui = row(U, uint32_t(i * DIM));  // U0 (stage 2 has not written U1 yet)
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) u0[k] = ui[k];
+                const double4 c4 = *row(reinterpret_cast<const double4*>(g.fcn), uint32_t(i));  // the FP64 centre
+                fcn = v3(c4.x, c4.y, c4.z);
+            }
+        }
+        if (!fast) {
             double un[DIM];
 #pragma unroll
             for (int k = 0; k < DIM; ++k) un[k] = (dt_here * double(src[k]) - coef * (sum[k] - c[k])) + u0[k];
@@ -659,8 +678,10 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             double wn[DIM];
             face_terms_mixed<DIM>(P, un, fcn, wn, vmax, dtx);
             store_w32<DIM>(W32_out, g.nf, i, wn);
+            if constexpr (STAGE == 2) {
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) row(U, uint32_t(i * DIM))[k] = un[k];
+                for (int k = 0; k < DIM; ++k) row(U, uint32_t(i * DIM))[k] = un[k];
+            }
         }
     }
     if constexpr (STAGE == 2) {
