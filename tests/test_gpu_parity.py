@@ -211,6 +211,26 @@ def test_mixed_discontinuous_state(kind, args, solver):
     assert l1 <= 1e-5
 
 
+def test_mixed_fast_rotation():
+    """Solid rotation at Omega = 1.5 (|v| up to 1.5, kinetic energy up to 30
+    times the pressure): most of stage 1's faces are formed from the FP64
+    state rather than by the FP32 pressure update, and the FP32-flux path
+    keeps the FP64 path's time and stop_check and meets the L1 bar."""
+    import splash_amd as S
+    mesh = S.quad_sphere(5)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=6, omega=(0, 0, 1.5))
+    f64 = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=S.FP64)
+    mx = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=S.MIXED)
+    for s in (f64, mx):
+        s.do_steps(1e-4, 100)
+    A, B = f64.get_state(), mx.get_state()
+    assert np.isfinite(B).all()
+    assert mx.time() == f64.time() and mx.get_stop_check() == f64.get_stop_check()
+    l1 = np.sum(np.abs(A[:, 0] - B[:, 0])) / np.sum(np.abs(A[:, 0]))
+    print(f"fast rotation quad(5) 100 steps: L1 rel density err {l1:.3e}, stop {f64.get_stop_check()}")
+    assert l1 <= 1e-5
+
+
 def test_c2_full_size_fp64_parity_and_conservation():
     """icosphere_hex(7) (163,842 faces): FP64 GPU vs oracle after 100 steps
     of the C2 workload, plus mass/energy/angular-momentum conservation."""
