@@ -57,14 +57,14 @@ from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state,
 # the bench line reports traffic as null.  The summary with the commands is
 # under profiles/.
 MEASURED_TRAFFIC = {
-    ("c3", "rotation03", "stage_1", "3ea616c3b6eee365"): 427_547_136,
-    ("c3", "rotation03", "stage_2", "3ea616c3b6eee365"): 604_047_155,
-    ("c5b", "harmonics", "stage_1", "3ea616c3b6eee365"): 1_003_534_131,
-    ("c5b", "harmonics", "stage_2", "3ea616c3b6eee365"): 1_279_338_496,
-    ("c5a", "harmonics", "stage_1", "3ea616c3b6eee365"): 583_163_187,
-    ("c5a", "harmonics", "stage_2", "3ea616c3b6eee365"): 798_029_312,
-    ("c2", "rotation05", "stage_1", "3ea616c3b6eee365"): 64_442_675,   # footprint fits the MALL:
-    ("c2", "rotation05", "stage_2", "3ea616c3b6eee365"): 81_057_997,   # L2-to-fabric bytes, not HBM
+    ("c3", "rotation03", "stage_1", "594b4e31064dec30"): 426_658_202,
+    ("c3", "rotation03", "stage_2", "594b4e31064dec30"): 603_291_750,
+    ("c5b", "harmonics", "stage_1", "594b4e31064dec30"): 1_003_492_864,
+    ("c5b", "harmonics", "stage_2", "594b4e31064dec30"): 1_278_797_824,
+    ("c5a", "harmonics", "stage_1", "594b4e31064dec30"): 583_310_131,
+    ("c5a", "harmonics", "stage_2", "594b4e31064dec30"): 797_970_125,
+    ("c2", "rotation05", "stage_1", "594b4e31064dec30"): 64_470_528,   # footprint fits the MALL:
+    ("c2", "rotation05", "stage_2", "594b4e31064dec30"): 81_107_558,   # L2-to-fabric bytes, not HBM
 }
 KERNEL_SOURCES = ["stage_mixed.hip", "stage_kernels.inc", "engine_types.hpp", "launch.hpp", "layout.cpp",
                   "run_setup.cpp"]

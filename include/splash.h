@@ -80,7 +80,8 @@ typedef struct splash_params {
                             * (OpenMP's default team); the tables do not depend on it */
     int32_t patch_faces; /* SPLASH_MIXED: faces per stage-kernel patch, 1..128 (above 64 two
                           * waves update a patch's faces); 0 = 112 for 256-thread workgroups on
-                          * meshes of mean valence at most 4.5, else 64 (per run in an ensemble);
+                          * meshes of mean valence at most 4.5, else 64 (per run in an ensemble;
+                          * 64 where the larger patches' stencil face sets would overflow LDS);
                           * -1 = the fastest of a few (block_threads, patch_faces) candidates,
                           * measured at creation */
     int32_t no_graph;    /* 0: splash_do_steps replays a captured HIP graph (captured after the

@@ -369,8 +369,14 @@ int layout_stats(const splash_mesh_desc* mesh, int32_t threads, int32_t faces, i
     return guarded([&] {
         const splash::PolyMesh m = from_desc(mesh);
         const splash::Geometry g = splash::build_geometry(m, 0);
-        const splash::Layout L = splash::build_layout(g, splash::mixed::slots(threads),
-                                                     faces ? faces : splash::default_patch_faces(m, threads));
+        splash::Layout L;
+        try {
+            L = splash::build_layout(g, splash::mixed::slots(threads),
+                                     faces ? faces : splash::default_patch_faces(m, threads));
+        } catch (const splash::FaceSetOverflow&) {
+            if (faces) throw;  // a defaulted cap falls back to one face wave, as the solvers do
+            L = splash::build_layout(g, splash::mixed::slots(threads), splash::kPatchFacesOneWave);
+        }
         splash::validate_layout(g, L);
         int32_t max_halo = 0, nonempty = 0;
         for (int32_t b = 0; b < L.n_blocks(); ++b) {

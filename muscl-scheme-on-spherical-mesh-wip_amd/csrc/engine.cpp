@@ -30,9 +30,10 @@ Engine::Engine(const PolyMesh& mesh, const double* U_in, int dim, int solver, do
         throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
     const int32_t threads = block_threads ? block_threads : mixed::default_threads();
     const int32_t cap = patch_faces > 0 ? patch_faces : default_patch_faces(mesh, threads);
+    // a defaulted cap falls back to one face wave where its face sets overflow
     RunHost r = precision == kMixed
                     ? build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, mixed::slots(threads), cap,
-                                     geometry_threads)
+                                     geometry_threads, patch_faces > 0 ? 0 : kPatchFacesOneWave)
                     : build_run_host(mesh, U_in, dim, solver, gam, omega_ns, params, fp64::block_size(),
                                      fp64::block_size(), geometry_threads);
     if (precision == kMixed && patch_faces == -1) autotune_patch_shape(r, U_in, 1e-4);
@@ -88,7 +89,11 @@ PatchShape autotune_patch_shape(RunHost& r, const double* U_in, double dt0) {
     PatchShape best{mixed::default_threads(), kPatchFacesOneWave};
     double best_t = 0;
     for (const PatchShape& c : kPatchShapeCandidates) {
-        relayout(r, mixed::slots(c.threads), c.faces);
+        try {
+            relayout(r, mixed::slots(c.threads), c.faces);
+        } catch (const FaceSetOverflow&) {
+            continue;  // this shape's face sets do not fit the LDS face slots
+        }
         Engine e(r, U_in, kMixed);
         const double t = time_trial([&](int n) { e.steps(dt0, n); }, [&] { e.synchronize(); }, 32);
         if (best_t == 0 || t < best_t) {

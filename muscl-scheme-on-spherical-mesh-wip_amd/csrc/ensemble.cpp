@@ -40,10 +40,12 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
     std::vector<const double*> U_in;
     for (const Run& run : runs) {
         if (!run.mesh) throw std::invalid_argument("null mesh");
-        // without a set cap each run takes its own mesh's default
+        // without a set cap each run takes its own mesh's default, falling
+        // back to one face wave where its face sets overflow
         const int32_t cap = patch_faces > 0 ? patch_faces : default_patch_faces(*run.mesh, threads);
         hosts.push_back(build_run_host(*run.mesh, run.U_in, dim, solver, run.gam, run.omega_ns, run.params,
-                                       mixed::slots(threads), cap, geometry_threads));
+                                       mixed::slots(threads), cap, geometry_threads,
+                                       patch_faces > 0 ? 0 : kPatchFacesOneWave));
         U_in.push_back(run.U_in);
     }
     if (patch_faces == -1) {
@@ -53,7 +55,11 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
         double best_t = 0;
         std::vector<double> dt0(runs.size(), 1e-4);
         for (const PatchShape& c : kPatchShapeCandidates) {
-            for (RunHost& h : hosts) relayout(h, mixed::slots(c.threads), c.faces);
+            try {
+                for (RunHost& h : hosts) relayout(h, mixed::slots(c.threads), c.faces);
+            } catch (const FaceSetOverflow&) {
+                continue;  // a run's face sets do not fit at this shape; the loop below re-lays every run
+            }
             Ensemble trial(hosts, U_in, dim, solver);
             const double t =
                 time_trial([&](int n) { trial.steps(dt0.data(), n); }, [&] { trial.synchronize(); }, 32);

@@ -257,7 +257,7 @@ Layout build_layout(const Geometry& g, int32_t slots, int32_t max_faces) {
             for (int32_t f : fs) local[f] = -1;
         }
     }
-    if (too_big) throw std::runtime_error("a block's stencil face set exceeds the LDS face slots");
+    if (too_big) throw FaceSetOverflow("a block's stencil face set exceeds the LDS face slots");
     L.fset_off.assign(size_t(nb) + 1, 0);
     for (int32_t b = 0; b < nb; ++b) L.fset_off[size_t(b) + 1] = L.fset_off[size_t(b)] + int32_t(sets[size_t(b)].size());
     L.fset_face.assign(size_t(L.fset_off[size_t(nb)]), 0);

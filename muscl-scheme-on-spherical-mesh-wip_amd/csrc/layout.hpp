@@ -6,11 +6,18 @@
 #pragma once
 
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 
 #include "geometry.hpp"
 
 namespace splash {
+
+// A patch whose stencil faces exceed the LDS face set (Layout::kFaceSetMax):
+// a smaller patch face cap lays the mesh out.
+struct FaceSetOverflow : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 struct Layout {
     int32_t slots = 0;                 // LDS half-edge slots per block

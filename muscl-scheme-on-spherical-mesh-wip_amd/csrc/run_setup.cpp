@@ -30,7 +30,8 @@ void light_observers(double obs[16]) {
 }
 
 RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-                       const RunParams& params, int32_t slots, int32_t max_faces, int threads) {
+                       const RunParams& params, int32_t slots, int32_t max_faces, int threads,
+                       int32_t fallback_faces) {
     if (solver < kHLLE || solver > kHLLCPLUS) throw std::invalid_argument("unknown solver");
     if ((solver == kHLLE && dim != 4) || (solver != kHLLE && dim != 5))
         throw std::invalid_argument("dim must be 4 for MUSCL_HLLE and 5 for the adiabatic solvers");
@@ -93,7 +94,12 @@ RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int so
     P.f.dmdt_coef = float(P.dmdt_coef);
     for (int c = 0; c < 3; ++c) P.f.om_acc[c] = float(P.om_acc[c]);
     r.params = params;
-    relayout(r, slots, max_faces);
+    try {
+        relayout(r, slots, max_faces);
+    } catch (const FaceSetOverflow&) {
+        if (fallback_faces <= 0 || fallback_faces >= max_faces) throw;
+        relayout(r, slots, fallback_faces);
+    }
     return r;
 }
 

@@ -42,9 +42,12 @@ struct RunHost {
 };
 
 // Checks the arguments (throws std::invalid_argument), builds the geometry
-// with `threads` OpenMP threads and the layout with the given patch shape.
+// with `threads` OpenMP threads and the layout with the given patch shape;
+// with fallback_faces > 0 a layout whose face sets overflow at max_faces is
+// built at fallback_faces instead.
 RunHost build_run_host(const PolyMesh& mesh, const double* U_in, int dim, int solver, double gam, double omega_ns,
-                       const RunParams& params, int32_t slots, int32_t max_faces, int threads);
+                       const RunParams& params, int32_t slots, int32_t max_faces, int threads,
+                       int32_t fallback_faces = 0);
 
 // Rebuilds the layout and every table in its order for another patch shape
 // (the geometry and physical parameters are kept).
