@@ -47,6 +47,16 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["finalize_dt", "stage_1", "stage_2"]
 
+
+def kernel_ms(prof, mixed):
+    """Per-launch mean ms of one step's kernels and of the whole step from
+    profile_step; the FP32-flux path launches no dt kernel (its stage 1 forms
+    dt), so it lists the two stage kernels only."""
+    names = KERNELS[1:] if mixed else KERNELS
+    out = {k: float(prof[KERNELS.index(k)]) for k in names}
+    out["step"] = float(prof[3])
+    return out
+
 from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state, make_mesh  # noqa: E402
 
 # HBM bytes per launch of the FP32-flux stage kernels, from rocprofv3 --pmc
@@ -57,14 +67,14 @@ from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state,
 # the bench line reports traffic as null.  The summary with the commands is
 # under profiles/.
 MEASURED_TRAFFIC = {
-    ("c3", "rotation03", "stage_1", "594b4e31064dec30"): 426_658_202,
-    ("c3", "rotation03", "stage_2", "594b4e31064dec30"): 603_291_750,
-    ("c5b", "harmonics", "stage_1", "594b4e31064dec30"): 1_003_492_864,
-    ("c5b", "harmonics", "stage_2", "594b4e31064dec30"): 1_278_797_824,
-    ("c5a", "harmonics", "stage_1", "594b4e31064dec30"): 583_310_131,
-    ("c5a", "harmonics", "stage_2", "594b4e31064dec30"): 797_970_125,
-    ("c2", "rotation05", "stage_1", "594b4e31064dec30"): 64_470_528,   # footprint fits the MALL:
-    ("c2", "rotation05", "stage_2", "594b4e31064dec30"): 81_107_558,   # L2-to-fabric bytes, not HBM
+    ("c3", "rotation03", "stage_1", "9859f6581d88ee90"): 431_713_280,
+    ("c3", "rotation03", "stage_2", "9859f6581d88ee90"): 603_089_510,
+    ("c5b", "harmonics", "stage_1", "9859f6581d88ee90"): 1_007_137_280,
+    ("c5b", "harmonics", "stage_2", "9859f6581d88ee90"): 1_281_803_776,
+    ("c5a", "harmonics", "stage_1", "9859f6581d88ee90"): 585_888_666,
+    ("c5a", "harmonics", "stage_2", "9859f6581d88ee90"): 799_204_966,
+    ("c2", "rotation05", "stage_1", "9859f6581d88ee90"): 64_669_184,   # footprint fits the MALL:
+    ("c2", "rotation05", "stage_2", "9859f6581d88ee90"): 81_592_934,   # L2-to-fabric bytes, not HBM
 }
 KERNEL_SOURCES = ["stage_mixed.hip", "stage_kernels.inc", "engine_types.hpp", "launch.hpp", "layout.cpp",
                   "run_setup.cpp"]
@@ -246,7 +256,7 @@ def ensemble_bench(S, steps, warmup, dt0, profile_steps):
         "one_run_at_a_time": {"mixed": seq["mixed"], "fp64": seq["fp64"]},
         "speedup_vs_fp64_runs": (F * steps / el) / seq["fp64"],
         "l1_err_vs_fp64": num / den, "linf_err_vs_fp64": linf,
-        "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
+        "kernel_ms": kernel_ms(prof, True),
         "stage_2_hbm_frac": ab / (prof[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "workgroups_per_stage": blocks, "runs_with_stop_check": int(stops),
         "patch_faces_tuned": tuned_faces, "block_threads_tuned": tuned_threads, "setup_s_incl_tuning": t_setup,
@@ -414,9 +424,9 @@ def main():
         "linf_err_vs_fp64": linf,
         "speedup_vs_fp64": value / value64,
         "fp64": {"value": value64, "ms_per_step": el64 / steps * 1e3,
-                 "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof64))),
+                 "kernel_ms": kernel_ms(prof64, False),
                  "step_hbm_frac": step_bytes / (prof64[3] * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms": dict(zip(KERNELS + ["step"], map(float, prof))),
+        "kernel_ms": kernel_ms(prof, True),
         "step_algorithmic_bytes": step_bytes,
         "step_achieved_GBs": step_bytes / (prof[3] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,

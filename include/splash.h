@@ -129,7 +129,9 @@ int splash_do_steps(splash_ctx *ctx, double dt0, int n);
 /* Measurement extension: runs n do_step(dt0) calls back to back on the
  * context's stream, each kernel launched with start/stop HIP events bound
  * to its dispatch, one wait at the end; ms_out[4] receives the mean
- * duration in ms of [dt finalize, stage 1, stage 2, whole step]. */
+ * duration in ms of [dt finalize, stage 1, stage 2, whole step]
+ * (SPLASH_MIXED launches no dt kernel: its stage 1 forms dt, and
+ * ms_out[0] = 0). */
 int splash_profile_step(splash_ctx *ctx, double dt0, int n, double *ms_out);
 /* Measurement extension (no counterpart in the reference): the HBM
  * bandwidth the library's own streaming copy kernels reach on the current
@@ -248,8 +250,8 @@ void splash_ensemble_destroy(splash_ensemble *e);
  * and later calls replay it. */
 int splash_ensemble_step(splash_ensemble *e, const double *dt0, int nsteps);
 /* Measurement extension, as splash_profile_step: n steps of every run with
- * dispatch-bound HIP events; ms_out[4] = mean [dt finalize, stage 1,
- * stage 2, whole step] of the ensemble's launches. */
+ * dispatch-bound HIP events; ms_out[4] = mean [dt finalize (0: stage 1
+ * forms dt), stage 1, stage 2, whole step] of the ensemble's launches. */
 int splash_ensemble_profile_step(splash_ensemble *e, const double *dt0, int n, double *ms_out);
 int splash_ensemble_synchronize(splash_ensemble *e);
 /* n_runs, total faces, total half-edges, stage-kernel workgroups */

@@ -177,12 +177,20 @@ void Engine::upload(const RunHost& r) {
         args_.W32b = static_cast<float*>(dev(sizeof(float) * row * nf, nullptr));
     }
     d.n_blocks = L.n_blocks();
-    d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
-    {
+    d.dt_partial = d.dt_partial2 = nullptr;
+    d.dt_ticket = nullptr;
+    d.dt_acc = nullptr;
+    if (precision_ == kFP64) {
+        // per-block partials and the dt kernel's own
+        d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
         const std::vector<double> p2(2 * size_t(fp64::finalize_blocks()), 0.0);
         const unsigned zero = 0;
         d.dt_partial2 = static_cast<double*>(dev(sizeof(double) * p2.size(), p2.data()));
         d.dt_ticket = static_cast<unsigned*>(dev(sizeof(unsigned), &zero));
+    } else {
+        // the two accumulator buffers (cleared again before each prepare)
+        const std::vector<unsigned long long> keys(size_t(kDtRunWords), ~0ull);
+        d.dt_acc = static_cast<unsigned long long*>(dev(sizeof(unsigned long long) * keys.size(), keys.data()));
     }
     d.blk_run = d.run_blk = nullptr;
     d.n_runs = 1;
@@ -231,9 +239,12 @@ void Engine::enqueue_prepare() {
 
 void Engine::enqueue_kernel(int which, double dt0, const KernelEvents* ev, const StepArgs* a) {
     const StepArgs& args = a ? *a : args_;
-    if (which == 0) fp64::enqueue_finalize_dt(args, dt0, stream_, ev);
-    else if (precision_ == kFP64) fp64::enqueue_stage(args, which, stream_, ev);
-    else mixed::enqueue_stage(args, which, stream_, ev);
+    if (precision_ == kFP64) {
+        if (which == 0) fp64::enqueue_finalize_dt(args, dt0, stream_, ev);
+        else fp64::enqueue_stage(args, which, stream_, ev);
+    } else if (which > 0) {
+        mixed::enqueue_stage(args, which, dt0, stream_, ev);  // stage 1 forms dt: no dt kernel
+    }
 }
 
 void Engine::enqueue_step(double dt0) {
@@ -245,23 +256,25 @@ void Engine::profile(double dt0, int n, double* ms) {
     // n steps launched back to back on this run's stream, each kernel with
     // start/stop HIP events bound to its own dispatch, one wait at the end
     constexpr int K = kKernelsPerStep;
-    const size_t nk = size_t(n > 0 ? n : 0) * K;
+    const int k0 = precision_ == kMixed ? 1 : 0;  // the mixed path launches no dt kernel
+    const int kn = K - k0;
+    const size_t nk = size_t(n > 0 ? n : 0) * size_t(kn);
     std::vector<hipEvent_t> ev(2 * nk, nullptr);
     double acc[K + 1] = {0};
     try {
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
         for (size_t q = 0; q < nk; ++q) {
             const KernelEvents ke{ev[2 * q], ev[2 * q + 1]};
-            enqueue_kernel(int(q % K), dt0, &ke);
+            enqueue_kernel(k0 + int(q % kn), dt0, &ke);
         }
         check(hipGetLastError(), "profile launch");
         if (nk) check(hipEventSynchronize(ev.back()), "hipEventSynchronize");
         for (size_t q = 0; q < nk; ++q) {
             float t = 0;
             check(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]), "hipEventElapsedTime");
-            acc[q % K] += t;
-            if (q % K == K - 1) {
-                check(hipEventElapsedTime(&t, ev[2 * (q - K + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
+            acc[k0 + q % kn] += t;
+            if (q % kn == size_t(kn - 1)) {
+                check(hipEventElapsedTime(&t, ev[2 * (q - kn + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
                 acc[K] += t;
             }
         }

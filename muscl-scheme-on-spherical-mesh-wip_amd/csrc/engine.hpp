@@ -50,7 +50,8 @@ public:
     void set_use_graph(bool on) { use_graph_ = on; }
     void synchronize();
     // Runs n eager steps with HIP events around each launch; ms receives the
-    // average duration of [finalize_dt, stage 1, stage 2, whole step].
+    // average duration of [finalize_dt, stage 1, stage 2, whole step] (the
+    // mixed path has no dt kernel: its stage 1 forms dt, and ms[0] = 0).
     static constexpr int kKernelsPerStep = 3;
     void profile(double dt0, int n, double* ms);
 

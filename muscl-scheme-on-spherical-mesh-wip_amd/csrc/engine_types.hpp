@@ -18,17 +18,33 @@ enum StopCode : int32_t {
 };
 
 // Device-resident per-run scalars.  The time step never leaves the GPU: the
-// previous step's last kernel leaves per-block partials of the CFL/friction
-// limits (GeomPtrs::dt_partial) and a one-block kernel turns them into dt.
+// FP64 path's last kernel of a step leaves per-block partials of the
+// CFL/friction limits (GeomPtrs::dt_partial) and a small kernel turns them
+// into dt; the mixed path folds them into a few accumulator words
+// (GeomPtrs::dt_acc) that its next stage-1 kernel reduces itself.
 struct Scalars {
     double dt;                    // clamped dt of the current step
     double t;                     // simulation time
     double dt0;                   // requested dt
     unsigned long long steps;
     int32_t stop;                 // StopCode bits
-    int32_t pad;
+    int32_t acc_buf;              // mixed path: the dt_acc buffer this step's stage 2 fills
     unsigned long long first_err; // (code << 40) | half-edge or face index, atomicMin
 };
+
+// Mixed path: the dt limits of a run are folded into kDtSlots accumulator
+// slots (patch b into slot b % kDtSlots), each an order-preserving key pair
+// {0.1 h0 / max(|v|, c_s), min friction dt} kept by atomic min; a run has
+// two buffers of slots, one filled by a step's stage 2 while the other is
+// read and then cleared by the next stage 1.  Each (run, buffer, slot) key
+// pair has 4 KiB to itself, so no two entries share a cache line, memory
+// channel or neighbourhood: measured on a 14,000-patch stage kernel, the 16
+// slots packed into 256 bytes made it 40% slower, and the two buffers of a
+// slot 256 bytes apart (slots 4 KiB apart) still cost 2% of a step, while
+// 4 KiB entries cost nothing measurable.  128 KiB per run.
+constexpr int kDtSlots = 16;
+constexpr int kDtEntryWords = 512;  // 4 KiB
+constexpr int kDtRunWords = 2 * kDtSlots * kDtEntryWords;
 
 struct PhysParams {
     int32_t dim, solver, accretion_on, friction_on;
@@ -95,9 +111,10 @@ struct GeomPtrs {
     int32_t fset_max;         // largest face set (dynamic LDS of the mixed kernel)
     int32_t slot_max;         // largest patch slot count, a multiple of 4 (dynamic LDS)
     int32_t n_blocks;         // multiple of 8 (trailing blocks may be empty)
-    double* dt_partial;       // [n_blocks][2] by logical block: max(|v|, c_s), min friction dt
-    double* dt_partial2;      // [n_runs][16][2]: the dt kernel's per-workgroup partials
+    double* dt_partial;       // [n_blocks][2] by logical block: max(|v|, c_s), min friction dt (FP64 path)
+    double* dt_partial2;      // [n_runs][16][2]: the dt kernel's per-workgroup partials (FP64 path)
     unsigned* dt_ticket;      // [n_runs]: the dt kernel's arrival counters (0 between launches)
+    unsigned long long* dt_acc;  // mixed path: [n_runs][2][kDtSlots][kDtEntryWords], a key pair per entry
     // ensembles: several runs' patches in one grid (null / 1 for one run)
     const int32_t* blk_run;   // [n_blocks] run of each logical block
     const int32_t* run_blk;   // [n_runs+1] block range of each run

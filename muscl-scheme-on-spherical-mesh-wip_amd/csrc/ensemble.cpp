@@ -187,11 +187,11 @@ void Ensemble::init(const std::vector<RunHost>& hosts, const std::vector<const d
         d.fset_max = fset_max;
         d.slot_max = slot_max;
         d.n_blocks = int32_t(blk_face.size()) - 1;
-        d.dt_partial = static_cast<double*>(dev(sizeof(double) * 2 * size_t(d.n_blocks), nullptr));
-        const std::vector<double> p2(2 * size_t(fp64::finalize_blocks()) * size_t(R), 0.0);
-        const std::vector<unsigned> tickets(size_t(R), 0u);
-        d.dt_partial2 = static_cast<double*>(dev(sizeof(double) * p2.size(), p2.data()));
-        d.dt_ticket = static_cast<unsigned*>(dev(sizeof(unsigned) * tickets.size(), tickets.data()));
+        // the mixed path's per-run dt accumulators (no dt kernel)
+        d.dt_partial = d.dt_partial2 = nullptr;
+        d.dt_ticket = nullptr;
+        const std::vector<unsigned long long> keys(size_t(R) * kDtRunWords, ~0ull);
+        d.dt_acc = static_cast<unsigned long long*>(dev(sizeof(unsigned long long) * keys.size(), keys.data()));
         d.blk_run = static_cast<const int32_t*>(dev(sizeof(int32_t) * blk_run.size(), blk_run.data()));
         d.run_blk = static_cast<const int32_t*>(dev(sizeof(int32_t) * run_blk.size(), run_blk.data()));
         d.n_runs = R;
@@ -274,8 +274,8 @@ void Ensemble::upload_dt0(const double* dt0) {
 }
 
 void Ensemble::enqueue_kernel(int which, const KernelEvents* ev) {
-    if (which == 0) fp64::enqueue_finalize_dt(args_, 0.0, stream_, ev);  // dt0 from the per-run array
-    else mixed::enqueue_stage(args_, which, stream_, ev);
+    // stage 1 forms each run's dt (dt0 from the per-run array): no dt kernel
+    if (which > 0) mixed::enqueue_stage(args_, which, 0.0, stream_, ev);
 }
 
 void Ensemble::enqueue_step() {
@@ -325,23 +325,24 @@ void Ensemble::profile(const double* dt0, int n, double* ms) {
     upload_dt0(dt0);
     readout_valid_ = false;
     constexpr int K = kKernelsPerStep;
-    const size_t nk = size_t(n > 0 ? n : 0) * K;
+    constexpr int k0 = 1, kn = K - k0;  // no dt kernel: the two stage launches
+    const size_t nk = size_t(n > 0 ? n : 0) * size_t(kn);
     std::vector<hipEvent_t> ev(2 * nk, nullptr);
     double acc[K + 1] = {0};
     try {
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
         for (size_t q = 0; q < nk; ++q) {
             const KernelEvents ke{ev[2 * q], ev[2 * q + 1]};
-            enqueue_kernel(int(q % K), &ke);
+            enqueue_kernel(k0 + int(q % kn), &ke);
         }
         check(hipGetLastError(), "profile launch");
         if (nk) check(hipEventSynchronize(ev.back()), "hipEventSynchronize");
         for (size_t q = 0; q < nk; ++q) {
             float t = 0;
             check(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]), "hipEventElapsedTime");
-            acc[q % K] += t;
-            if (q % K == K - 1) {
-                check(hipEventElapsedTime(&t, ev[2 * (q - K + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
+            acc[k0 + q % kn] += t;
+            if (q % kn == size_t(kn - 1)) {
+                check(hipEventElapsedTime(&t, ev[2 * (q - kn + 1)], ev[2 * q + 1]), "hipEventElapsedTime");
                 acc[K] += t;
             }
         }

@@ -42,7 +42,7 @@ public:
     void synchronize();
     static constexpr int kKernelsPerStep = 3;
     // n back-to-back steps, each kernel timed with dispatch-bound HIP events:
-    // ms = mean [dt finalize, stage 1, stage 2, whole step]
+    // ms = mean [dt finalize (0: stage 1 forms dt), stage 1, stage 2, whole step]
     void profile(const double* dt0, int n, double* ms);
 
     int n_runs() const { return int(faces_.size()); }

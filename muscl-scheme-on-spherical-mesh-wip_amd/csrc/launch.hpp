@@ -69,8 +69,13 @@ void enqueue_set_scalar(double* p, double v, hipStream_t s);
 }  // namespace fp64
 
 namespace mixed {
+// the stage inputs of the current state and its dt limits (which the next
+// stage 1 turns into dt)
 void enqueue_prepare(const StepArgs& a, hipStream_t s);
-void enqueue_stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev = nullptr);
+// one stage of a step: there is no separate dt kernel; stage 1 forms dt
+// from dt0 (a.dt0_dev[run] when set, else the argument) and the limits the
+// previous stage 2 or enqueue_prepare left
+void enqueue_stage(const StepArgs& a, int stage, double dt0, hipStream_t s, const KernelEvents* ev = nullptr);
 int slots(int threads);  // LDS half-edge slots per block of `threads` threads
 int max_faces();         // faces per block
 int default_threads();   // threads per block unless set or tuned

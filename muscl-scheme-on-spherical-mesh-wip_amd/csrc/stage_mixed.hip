@@ -57,9 +57,13 @@ __device__ __forceinline__ const T* row(const T* base, uint32_t idx) {
 // The dt limits of a patch's two face waves, combined in LDS without a
 // barrier: each wave folds its max(|v|, c_s) and min friction dt into the
 // block's LDS words with workgroup-scope atomics, and the second wave to
-// arrive writes the block's partial.  vmax > 0 orders like its bit pattern;
-// dtx (any sign, never NaN: the comparisons skip NaN candidates) is mapped to
-// an unsigned key of the same order.
+// arrive folds the block's limits into its run's accumulator slot with
+// device-scope atomic minima.  The CFL bound enters as the quotient
+// 0.1 h0 / max(|v|, c_s) itself: a correctly rounded quotient is monotone in
+// the divisor, so the minimum over patches equals the quotient of the
+// maximum bit for bit.  vmax > 0 orders like its bit pattern; dtx (any sign,
+// never NaN: the comparisons skip NaN candidates) and the quotient are mapped
+// to unsigned keys of the same order.
 struct DtCombine {
     unsigned long long v, x;
     unsigned arrived;
@@ -76,15 +80,43 @@ __device__ __forceinline__ void dt_combine_init(DtCombine* c) {
     c->x = order_key(1e20);
     c->arrived = 0;
 }
-// lane 0 of each of the two face waves, once per block
-__device__ __forceinline__ void dt_combine(DtCombine* c, double* partial, int b, double vmax, double dtx) {
+static_assert(kDtSlots == 16, "the first 16 lanes (one DPP row) reduce the dt slots");
+// slot s of buffer buf of a run's dt accumulators: {CFL key, friction key}
+__device__ __forceinline__ unsigned long long* dt_slot(const GeomPtrs& g, int run, int buf, int s) {
+    return g.dt_acc + ((size_t(run) * 2 + size_t(buf)) * kDtSlots + size_t(s)) * kDtEntryWords;
+}
+// The minimum of a 64-bit key over each row of 16 lanes, through DPP lane
+// moves (no LDS round trip): swaps within quads, then row rotations by 4
+// and 8.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_move(unsigned long long v) {
+    const int lo = __builtin_amdgcn_mov_dpp(int(unsigned(v)), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(unsigned(v >> 32)), CTRL, 0xF, 0xF, false);
+    return (unsigned long long)(unsigned(hi)) << 32 | unsigned(lo);
+}
+__device__ __forceinline__ unsigned long long row_min(unsigned long long v) {
+    unsigned long long o = dpp_move<0xB1>(v);  // quad_perm [1,0,3,2]: lane ^ 1
+    v = o < v ? o : v;
+    o = dpp_move<0x4E>(v);  // quad_perm [2,3,0,1]: lane ^ 2
+    v = o < v ? o : v;
+    o = dpp_move<0x124>(v);  // row_ror:4
+    v = o < v ? o : v;
+    o = dpp_move<0x128>(v);  // row_ror:8
+    return o < v ? o : v;
+}
+// lane 0 of each of the two face waves, once per block; acc = the block's
+// accumulator slot
+__device__ __forceinline__ void dt_combine(DtCombine* c, unsigned long long* acc, double h0, double vmax,
+                                           double dtx) {
     __hip_atomic_fetch_max(&c->v, (unsigned long long)(__double_as_longlong(vmax)), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_fetch_min(&c->x, order_key(dtx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (__hip_atomic_fetch_add(&c->arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1u) {
-        partial[2 * b] = __longlong_as_double(
-            (long long)(__hip_atomic_load(&c->v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)));
-        partial[2 * b + 1] = from_order_key(__hip_atomic_load(&c->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        const double v =
+            __longlong_as_double((long long)(__hip_atomic_load(&c->v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)));
+        const unsigned long long kx = __hip_atomic_load(&c->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_min(acc, order_key(h0 * 0.1 / v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_min(acc + 1, kx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -142,8 +174,8 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
 }
 
 // find_U_edges for one half-edge in FP32 (MUSCL_base.hpp:375-433) from the
-// block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed; sA:
-// dt / area): li = owner, plus0, plus1, minus0, minus1 local indices; hs =
+// block's LDS face states (sW: Sigma, L; sP: pressure; sC: sound speed; sIA:
+// 1 / area) and the step's dt32: li = owner, plus0, plus1, minus0, minus1 local indices; hs =
 // {H+/H-, BM/H+, beta+0, beta-0} (each pair's second weight is 1 - the
 // first); arc = the edge's arc length; ec = the unit edge midpoint; en = the
 // half-edge's outward normal.
@@ -152,8 +184,8 @@ __device__ __forceinline__ void limiter32(const float* D, float rho, float e, fl
 // w + d+ (BM/H+) lim.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_lds(const float4* sW, const float* sP, const float* sC,
-                                                const float* sA, const int* li, int valence, float4 hs, float arc,
-                                                V3<float> ec, V3<float> en, float* out) {
+                                                const float* sIA, float dt32, const int* li, int valence, float4 hs,
+                                                float arc, V3<float> ec, V3<float> en, float* out) {
     const float bp = hs.z, bm = hs.w;
     float w[5][DIM];
 #pragma unroll
@@ -174,7 +206,7 @@ __device__ __forceinline__ void reconstruct_lds(const float4* sW, const float* s
     // (ec x L) / -Sigma
     const float cl = dot(cross(ec, v3(wi[1], wi[2], wi[3])), en);
     const float e = __builtin_amdgcn_rcpf(hs.y);  // H+/BM
-    const float cfl = arc * sA[li[0]];            // dt arc / area
+    const float cfl = arc * (dt32 * sIA[li[0]]);  // dt arc / area
     if constexpr (DIM == 5) {
         const float nu = (sC[li[0]] - cl * __builtin_amdgcn_rcpf(wi[0])) * cfl;
         limiter32<DIM>(D, hs.x, e, nu, valence, lim);
@@ -428,24 +460,27 @@ __device__ __forceinline__ SlotRows load_slot(const GeomPtrs& g, uint32_t ext, u
 // valence << 28.
 template <int DIM>
 __device__ __forceinline__ void reconstruct_slot(const float4* sW, const float* sP, const float* sC,
-                                                 const float* sA, const SlotRows& r, float* out) {
+                                                 const float* sIA, float dt32, const SlotRows& r, float* out) {
     const uint2 ra = r.ra;
     const int li[5] = {int((ra.y >> 10) & 1023u), int(ra.x & 1023u), int((ra.x >> 10) & 1023u),
                        int((ra.x >> 20) & 1023u), int(ra.y & 1023u)};
     const float sg = (ra.x >> 30) & 1u ? -1.0f : 1.0f;
-    reconstruct_lds<DIM>(sW, sP, sC, sA, li, int(ra.y >> 28), r.hs, r.e4.w, v3(r.e4.x, r.e4.y, r.e4.z),
+    reconstruct_lds<DIM>(sW, sP, sC, sIA, dt32, li, int(ra.y >> 28), r.hs, r.e4.w, v3(r.e4.x, r.e4.y, r.e4.z),
                          v3(sg * r.n4.x, sg * r.n4.y, sg * r.n4.z), out);
 }
 
+// dt0_arg / dt0v: the requested dt (dt0v[run] when dt0v is set), read by
+// stage 1, which forms the step's dt.
 template <int DIM, int SOLVER, int STAGE, int NT>
 __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams* __restrict__ Pv,
                                                        Scalars* __restrict__ scv, double* __restrict__ U,
                                                        const float* __restrict__ W32,
-                                                       float* __restrict__ W32_out) {
+                                                       float* __restrict__ W32_out, double dt0_arg,
+                                                       const double* __restrict__ dt0v) {
     // dynamic LDS: the reconstructed half-edge states [slot_max][DIM] (after
     // the flux phase has read them, the same rows hold each own slot's flux
     // times chord length), then the face set's FP32 states (float4 Sigma, L
-    // rows; dim 5: pressures, sound speeds) and dt / area, each sized by the
+    // rows; dim 5: pressures, sound speeds) and 1 / area, each sized by the
     // layout
     extern __shared__ float4 lds_dyn[];
     float* const sUP = reinterpret_cast<float*>(lds_dyn);
@@ -453,7 +488,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     float4* const sW = lds_dyn + (g.slot_max * DIM) / 4;     // slot_max is a multiple of 4
     float* const sP = reinterpret_cast<float*>(sW + g.fset_max);  // dim 5: pressure
     float* const sC = sP + g.fset_max;                              // dim 5: sound speed
-    float* const sA = DIM == 5 ? sC + g.fset_max : sP;              // dt / area
+    float* const sIA = DIM == 5 ? sC + g.fset_max : sP;             // 1 / area
     const int b = xcd_block();
     const int run = g.blk_run ? g.blk_run[b] : 0;  // ensembles: this patch's run
     const PhysParams P = Pv[run];
@@ -464,15 +499,33 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     const int hoff = g.halo_off[b];
     const int n_slots = n_own + (g.halo_off[b + 1] - hoff);
     const int t = threadIdx.x;
-    const double dt = sc->dt;
-    const float dt32 = float(dt);
     const uint32_t ext0 = uint32_t(hbase + hoff), edge_base = ext0 >> 1;
+    const bool first_patch = b == (g.run_blk ? g.run_blk[run] : 0);
 
     const int i = f0 + t;
     const bool face = i < f1;  // t < kFaces: waves 0 and 1
 
+    // The step's dt.  Stage 2 reads it from the scalars.  Stage 1 forms it
+    // (MUSCL_base.hpp:100-115, adiabatic.hpp:543-580) from the run's
+    // accumulator buffer that the previous stage 2, or the prepare kernel,
+    // filled: wave 0 of every block reduces the kDtSlots slots (the same
+    // keys give every block the same dt) after its phase-0 gathers, while
+    // the other waves finish theirs, and the run's first patch records dt,
+    // raises the negative-dt stop, clears the other buffer for this step's
+    // stage 2 and names it in the scalars.
     __shared__ DtCombine s_dt;
-    if (STAGE == 2 && t == 0) dt_combine_init(&s_dt);  // before the first barrier
+    __shared__ double s_dt_step;
+    double dt = 0.0;
+    int acc_rd = 0, acc_wr = 0;
+    ulonglong2 acc_k = make_ulonglong2(~0ull, ~0ull);
+    if constexpr (STAGE == 1) {
+        acc_rd = int(sc->steps & 1ull);
+        if (t < kDtSlots) acc_k = *reinterpret_cast<const ulonglong2*>(dt_slot(g, run, acc_rd, t));
+    } else {
+        dt = sc->dt;
+        acc_wr = sc->acc_buf & 1;
+        if (t == 0) dt_combine_init(&s_dt);  // before the first barrier
+    }
 
     // phase 0: stage the FP32 states of the block's face set in LDS (own
     // faces first, so face f0 + t sits at local index t); the first slot's
@@ -498,7 +551,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             if (q < n_fs) {
                 const float4 w = *row(W4, f[j]);
                 sW[q] = w;
-                sA[q] = dt32 * *row(g.inv_area, f[j]);
+                sIA[q] = *row(g.inv_area, f[j]);
                 if constexpr (DIM == 5) {
                     const float pq = *row(W32, uint32_t(4 * g.nf) + f[j]);
                     sP[q] = pq;
@@ -507,7 +560,36 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             }
         }
     }
+    if constexpr (STAGE == 1) {
+        if (t < 64) {  // wave 0, every lane active (row_min)
+            const unsigned long long kc = row_min(acc_k.x), kx = row_min(acc_k.y);
+            if (t == 0) {
+                const double dt0 = dt0v ? dt0v[run] : dt0_arg;
+                double d = dt0;
+                const double dt_cfl = from_order_key(kc);  // 0.1 h0 / max(|v|, c_s)
+                if (d > dt_cfl) d = dt_cfl;
+                if constexpr (DIM == 5) {
+                    double extra = from_order_key(kx);
+                    if (extra < 0) {
+                        if (first_patch) atomicOr(&sc->stop, splash::kStopNegativeDt);
+                        extra = 1e20;
+                    }
+                    if (d > extra) d = extra;
+                }
+                s_dt_step = d;
+                if (first_patch) {
+                    sc->dt0 = dt0;
+                    sc->dt = d;
+                    sc->acc_buf = acc_rd ^ 1;
+                }
+            }
+            if (first_patch && t < kDtSlots)
+                *reinterpret_cast<ulonglong2*>(dt_slot(g, run, acc_rd ^ 1, t)) = make_ulonglong2(~0ull, ~0ull);
+        }
+    }
     __syncthreads();
+    if constexpr (STAGE == 1) dt = s_dt_step;
+    const float dt32 = float(dt);
 
     // phase 1: reconstruct own and halo half-edges into LDS, slots t and
     // t + NT; thread k then requests the rows of edge k for phase 2
@@ -516,8 +598,8 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
         const int t1 = t + NT;
         SlotRows r1;
         if (t1 < n_slots) r1 = load_slot(g, ext0 + uint32_t(t1), edge_base);
-        if (t < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r0, &sUP[t * DIM]);
-        if (t1 < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sA, r1, &sUP[t1 * DIM]);
+        if (t < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sIA, dt32, r0, &sUP[t * DIM]);
+        if (t1 < n_slots) reconstruct_slot<DIM>(sW, sP, sC, sIA, dt32, r1, &sUP[t1 * DIM]);
     }
     // face data of phase 3, requested here so the flux phase hides the
     // loads without holding them through the reconstruction.  Stage 1's
@@ -595,7 +677,7 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
         double coef;     // round_diff / area * dt_here
         const double dt_here = (STAGE == 1) ? dt / 2 : dt;
         if constexpr (STAGE == 1) {
-            coef = 0.5 * P.round_diff * double(sA[t]);  // sA = dt / area in FP32
+            coef = 0.5 * P.round_diff * double(dt32 * sIA[t]);  // dt / area in FP32
         } else {
             // U0 is first needed after the source and the edge sum, which
             // hide this load
@@ -698,8 +780,8 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
                 if (ov > vmax) vmax = ov;
                 if (dtx > ox) dtx = ox;
             }
-            if ((t & 63) == 0) dt_combine(&s_dt, g.dt_partial, b, vmax, dtx);
-            if (t == 0 && b == (g.run_blk ? g.run_blk[run] : 0)) {  // the run's first patch advances its clock
+            if ((t & 63) == 0) dt_combine(&s_dt, dt_slot(g, run, acc_wr, b % kDtSlots), P.h0, vmax, dtx);
+            if (t == 0 && first_patch) {  // the run's first patch advances its clock
                 sc->t += sc->dt;
                 sc->steps += 1;
             }
@@ -710,38 +792,49 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
 namespace {
 
 template <int DIM, int SOLVER, int NT>
-void stage_nt(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
+void stage_nt(const StepArgs& a, int stage, double dt0, hipStream_t s, const KernelEvents* ev) {
     const size_t lds = size_t(a.geom.fset_max) * (DIM == 5 ? 28 : 20) + sizeof(float) * DIM * size_t(a.geom.slot_max);
     if (a.geom.fset_max > 1024 || a.geom.slot_max > 2 * NT || a.geom.slot_max % 4 || lds > 64 * 1024)
         throw std::runtime_error("stage block LDS exceeds the limit");
     if (stage == 1)
         launch(k_stage32<DIM, SOLVER, 1, NT>, a.geom.n_blocks, NT, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U, a.W32,
-               a.W32b);
+               a.W32b, dt0, a.dt0_dev);
     else
         launch(k_stage32<DIM, SOLVER, 2, NT>, a.geom.n_blocks, NT, lds, s, ev, a.geom, a.phys_dev, a.sc, a.U, a.W32b,
-               a.W32);
+               a.W32, dt0, a.dt0_dev);
 }
 
 template <int DIM, int SOLVER>
-void stage(const StepArgs& a, int stage, hipStream_t s, const KernelEvents* ev) {
+void stage(const StepArgs& a, int stage, double dt0, hipStream_t s, const KernelEvents* ev) {
     switch (a.block_threads) {
-    case 128: stage_nt<DIM, SOLVER, 128>(a, stage, s, ev); break;
-    case 192: stage_nt<DIM, SOLVER, 192>(a, stage, s, ev); break;
-    case 256: stage_nt<DIM, SOLVER, 256>(a, stage, s, ev); break;
+    case 128: stage_nt<DIM, SOLVER, 128>(a, stage, dt0, s, ev); break;
+    case 192: stage_nt<DIM, SOLVER, 192>(a, stage, dt0, s, ev); break;
+    case 256: stage_nt<DIM, SOLVER, 256>(a, stage, dt0, s, ev); break;
     default: throw std::invalid_argument("unsupported stage-kernel block size");
     }
 }
 
 }  // namespace
 
-// E-form state -> FP32 stage input and this state's dt partials, with the
-// arithmetic of k_stage32's epilogue (face_terms_mixed), so a run started or
-// resumed from a state continues exactly as one that reached it by stepping.
+// Every run's two dt accumulator buffers set to the neutral key (a kernel,
+// not a memset, so no memset node can enter a captured graph).
+__global__ __launch_bounds__(256) void k_clear_dt_acc(unsigned long long* __restrict__ acc, int n) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) acc[k] = ~0ull;
+}
+
+// E-form state -> FP32 stage input and this state's dt limits (into the
+// accumulator buffer the next stage 1 reads), with the arithmetic of
+// k_stage32's epilogue (face_terms_mixed), so a run started or resumed from
+// a state continues exactly as one that reached it by stepping.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_prepare32(GeomPtrs g, const PhysParams* __restrict__ Pv,
-                                                      const double* __restrict__ U, float* __restrict__ W32) {
+                                                      const Scalars* __restrict__ scv, const double* __restrict__ U,
+                                                      float* __restrict__ W32) {
     const int b = xcd_block();
-    const PhysParams P = Pv[g.blk_run ? g.blk_run[b] : 0];
+    const int run = g.blk_run ? g.blk_run[b] : 0;
+    const PhysParams P = Pv[run];
+    const int acc_buf = int(scv[run].steps & 1ull);
     const int t = threadIdx.x;
     __shared__ DtCombine s_dt;
     if (t == 0) dt_combine_init(&s_dt);
@@ -765,12 +858,14 @@ __global__ __launch_bounds__(kBlock) void k_prepare32(GeomPtrs g, const PhysPara
         if (ov > vmax) vmax = ov;
         if (dtx > ox) dtx = ox;
     }
-    if ((t & 63) == 0) dt_combine(&s_dt, g.dt_partial, b, vmax, dtx);
+    if ((t & 63) == 0) dt_combine(&s_dt, dt_slot(g, run, acc_buf, b % kDtSlots), P.h0, vmax, dtx);
 }
 
 void enqueue_prepare(const StepArgs& a, hipStream_t s) {
-    if (a.phys.dim == 5) k_prepare32<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W32);
-    else k_prepare32<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.U, a.W32);
+    const int n_acc = a.geom.n_runs * kDtRunWords;
+    k_clear_dt_acc<<<(n_acc + 255) / 256, 256, 0, s>>>(a.geom.dt_acc, n_acc);
+    if (a.phys.dim == 5) k_prepare32<5><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.sc, a.U, a.W32);
+    else k_prepare32<4><<<a.geom.n_blocks, kBlock, 0, s>>>(a.geom, a.phys_dev, a.sc, a.U, a.W32);
 }
 
 int slots(int threads) { return 2 * threads; }
@@ -782,12 +877,12 @@ bool valid_threads(int threads) {
     return false;
 }
 
-void enqueue_stage(const StepArgs& a, int st, hipStream_t s, const KernelEvents* ev) {
+void enqueue_stage(const StepArgs& a, int st, double dt0, hipStream_t s, const KernelEvents* ev) {
     switch (a.phys.solver) {
-    case kHLLE: stage<4, kHLLE>(a, st, s, ev); break;
-    case kHLLE_P: stage<5, kHLLE_P>(a, st, s, ev); break;
-    case kHLLC: stage<5, kHLLC>(a, st, s, ev); break;
-    default: stage<5, kHLLCPLUS>(a, st, s, ev); break;
+    case kHLLE: stage<4, kHLLE>(a, st, dt0, s, ev); break;
+    case kHLLE_P: stage<5, kHLLE_P>(a, st, dt0, s, ev); break;
+    case kHLLC: stage<5, kHLLC>(a, st, dt0, s, ev); break;
+    default: stage<5, kHLLCPLUS>(a, st, dt0, s, ev); break;
     }
 }
 

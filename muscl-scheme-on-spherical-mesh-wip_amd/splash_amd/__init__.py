@@ -184,7 +184,8 @@ class _MUSCL:
     def profile_step(self, dt0, n):
         """Mean ms per launch of [dt finalize, stage 1, stage 2, whole step]
         over n steps launched back to back on the run's stream, each kernel
-        with start/stop HIP events bound to its dispatch."""
+        with start/stop HIP events bound to its dispatch (MIXED launches no
+        dt kernel: its stage 1 forms dt, and the first entry is 0)."""
         out = np.zeros(4)
         check(lib().splash_profile_step(self._ctx, dt0, int(n), _dp(out)))
         return out

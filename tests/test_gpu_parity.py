@@ -120,18 +120,53 @@ def test_stop_check_and_negative_dt_match_oracle():
     assert bits & 8
 
 
-def test_graph_replay_equals_eager():
+def test_mixed_dt_limits_match_fp64_and_oracle():
+    """The mixed path forms dt inside its stage-1 kernel from limits its
+    stage 2 (or the prepare kernel) folded into accumulator keys.  With the
+    dt clamp active (dt0 = 1 on hex(5)) its first dt equals the FP64
+    path's and the oracle's, both from the same initial state (the mixed
+    path's |v| and c_s differ only by rounding), at every later step it
+    stays clamped, and a negative friction limit (Omega = 0.5) raises the
+    same stop bit without clamping dt, as on the reference build."""
+    import splash_amd as S
+    mesh = S.icosphere_hex(5)
+    U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=3, omega=(0, 0, 0.3))
+    mixed = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=S.MIXED)
+    gpu, ref = _pair(S, "hex", (5,), "HLLCplus", U0, 1, 1.0)
+    mixed.do_step(1.0)
+    assert ref.time() < 1e-2 and gpu.time() == pytest.approx(ref.time(), rel=1e-14)
+    assert mixed.time() == pytest.approx(gpu.time(), rel=1e-13)
+    assert not mixed.get_stop_check()
+    for _ in range(4):
+        mixed.do_step(1.0)
+        assert mixed.last_dt() < 1e-2
+    U1 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=2, omega=(0, 0, 0.5))
+    mixed = S.MUSCL_HLLCplus(mesh, U1, 5, GAM, 0.0, precision=S.MIXED)
+    gpu, ref = _pair(S, "hex", (5,), "HLLCplus", U1, 1, 1e-4)
+    mixed.do_step(1e-4)
+    assert ref.get_stop_check() and gpu.get_stop_check() and mixed.get_stop_check()
+    assert mixed.stop_bits()[0] & 8
+    assert mixed.time() == pytest.approx(ref.time(), rel=1e-13)
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_graph_replay_equals_eager(precision):
+    """Graph replays equal eager steps; for the mixed path the replays also
+    carry its dt (formed in stage 1 from the accumulator buffer the previous
+    stage 2 filled, the buffer chosen from the device step count), so any
+    mix of replays and eager steps must keep the buffers in step."""
     import splash_amd as S
     mesh = S.icosphere_hex(4)
     U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=5, omega=(0, 0, 0.2))
-    a = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0)
-    b = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0)
+    prec = getattr(S, precision)
+    a = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
+    b = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
     for _ in range(3):
-        a.do_steps(1e-4, 19)  # two 8-step graph replays + 3 eager steps each time
+        a.do_steps(1.0, 19)  # two 8-step graph replays + 3 eager steps each time; dt clamped
     for _ in range(57):
-        b.do_step(1e-4)
+        b.do_step(1.0)
     assert np.array_equal(a.get_state(), b.get_state())
-    assert a.time() == b.time()
+    assert a.time() == b.time() and a.last_dt() < 1e-2
 
 
 def test_mixed_precision_error_and_conservation():
@@ -373,7 +408,8 @@ def test_hex9_fp64_parity_with_split_dt_reduction():
 def test_profile_step_times_back_to_back_steps():
     """profile_step advances exactly n steps, gives positive per-kernel means
     that add up to the step time, and leaves the state where as many plain
-    steps leave it."""
+    steps leave it.  The mixed path launches no dt kernel (its stage 1 forms
+    dt), so its first mean is 0."""
     import splash_amd as S
     mesh = S.icosphere_hex(4)
     U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=5, omega=(0, 0, 0.2))
@@ -385,7 +421,10 @@ def test_profile_step_times_back_to_back_steps():
             b.do_step(1e-4)
         assert a.time() == b.time()
         assert np.array_equal(a.get_state(), b.get_state())
-        assert np.all(ms > 0) and ms[:3].sum() <= ms[3] * 1.0001
+        launched = ms[1:] if prec == S.MIXED else ms
+        assert np.all(launched > 0) and ms[:3].sum() <= ms[3] * 1.0001
+        if prec == S.MIXED:
+            assert ms[0] == 0
 
 
 @pytest.mark.parametrize("solver,kind,args,sources", [
