@@ -67,14 +67,14 @@ from workloads import ACCRETION, GAM, STATES, WORKLOADS, c4_runs, initial_state,
 # the bench line reports traffic as null.  The summary with the commands is
 # under profiles/.
 MEASURED_TRAFFIC = {
-    ("c3", "rotation03", "stage_1", "9859f6581d88ee90"): 431_713_280,
-    ("c3", "rotation03", "stage_2", "9859f6581d88ee90"): 603_089_510,
-    ("c5b", "harmonics", "stage_1", "9859f6581d88ee90"): 1_007_137_280,
-    ("c5b", "harmonics", "stage_2", "9859f6581d88ee90"): 1_281_803_776,
-    ("c5a", "harmonics", "stage_1", "9859f6581d88ee90"): 585_888_666,
-    ("c5a", "harmonics", "stage_2", "9859f6581d88ee90"): 799_204_966,
-    ("c2", "rotation05", "stage_1", "9859f6581d88ee90"): 64_669_184,   # footprint fits the MALL:
-    ("c2", "rotation05", "stage_2", "9859f6581d88ee90"): 81_592_934,   # L2-to-fabric bytes, not HBM
+    ("c3", "rotation03", "stage_1", "9fe017f0c5be1ffd"): 432_350_618,
+    ("c3", "rotation03", "stage_2", "9fe017f0c5be1ffd"): 603_728_486,
+    ("c5b", "harmonics", "stage_1", "9fe017f0c5be1ffd"): 1_008_276_070,
+    ("c5b", "harmonics", "stage_2", "9fe017f0c5be1ffd"): 1_281_739_571,
+    ("c5a", "harmonics", "stage_1", "9fe017f0c5be1ffd"): 586_104_525,
+    ("c5a", "harmonics", "stage_2", "9fe017f0c5be1ffd"): 798_778_778,
+    ("c2", "rotation05", "stage_1", "9fe017f0c5be1ffd"): 65_029_837,   # footprint fits the MALL:
+    ("c2", "rotation05", "stage_2", "9fe017f0c5be1ffd"): 81_832_448,   # L2-to-fabric bytes, not HBM
 }
 KERNEL_SOURCES = ["stage_mixed.hip", "stage_kernels.inc", "engine_types.hpp", "launch.hpp", "layout.cpp",
                   "run_setup.cpp"]

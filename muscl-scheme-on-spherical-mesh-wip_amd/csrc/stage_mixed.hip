@@ -26,13 +26,13 @@ __device__ __forceinline__ float m_pow(float x, float y) {
 // The mixed-precision stage kernel.  It follows k_stage (patches of faces,
 // LDS slots for their half-edges and halo, one face loop) but works on FP32
 // copies: the stage input as one float4 (Sigma, L) row per face plus (dim 5)
-// one pressure per face, staged per block in LDS with dt / area; per slot
-// an 8-byte packed record (stencil face-set indices, owner, edge, valence)
-// and a float4 {H+/H-, BM/H+, beta+0, beta-0}; per edge a float4 {ec, arc},
-// a float4 {en, chord} and a 4-byte slot record.  The state, the RK combine, the dt limits and
-// reductions and the flux accumulation (Kahan-compensated) stay FP64; the
-// accretion / cooling / friction sources run in FP32 on the FP32 stage
-// input.
+// one pressure per face, staged per block in LDS with 1 / area; per slot an
+// 8-byte packed record (stencil face-set indices, owner, edge, valence) and a
+// float4 {H+/H-, BM/H+, beta+0, beta-0}; per edge a float4 {ec, arc}, a
+// float4 {en, chord} and a 4-byte slot record.  The state, the RK combine,
+// the dt limits and reductions and the flux accumulation (Kahan-compensated
+// where it updates the state) stay FP64; the accretion / cooling / friction
+// sources run in FP32 on the FP32 stage input.
 
 // Patches: at most kFaces faces (waves 0 and 1 update them, 64 each) whose
 // own half-edges plus halo fit 2 * NT LDS slots, two slots per thread of an
@@ -669,8 +669,8 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
     }
     __syncthreads();
 
-    // phase 3: res2d (compensated FP64 sum of the edge terms), source, RK
-    // combine and density floor for one face per thread
+    // phase 3: res2d (FP64 sum of the edge terms, compensated in stage 2),
+    // source, RK combine and density floor for one face per thread
     double vmax = 1e-8, dtx = 1e20;
     if (face) {
         double u0[DIM];  // the FP64 state (stage 2, and stage 1's faces formed from it)
@@ -693,10 +693,13 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
             // on the face's stage input (p-form), from LDS
             if (P.accretion_on) source32(P, sW[t], sP[t], v3(float(fcn.x), float(fcn.y), float(fcn.z)), rate, src);
         }
-        // Kahan-compensated FP64 sum of the FP32 edge terms: c carries the
-        // (negated) low part each addition loses into the next term.  The
-        // terms are FP32 numbers, so the partial sums are exact, and c stays
-        // 0, unless a face's terms span more than 2^26 in magnitude.
+        // FP64 sum of the FP32 edge terms.  The terms are FP32 numbers, so
+        // the partial sums are exact unless a face's terms span more than
+        // 2^26 in magnitude.  Stage 2, which updates the FP64 state, adds
+        // them Kahan-compensated (c carries the negated low part each
+        // addition loses into the next term); stage 1's sums are rounded to
+        // FP32, far coarser than anything the compensation could recover,
+        // so it adds them plainly (c stays 0).
         double sum[DIM], c[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) sum[k] = c[k] = 0;
@@ -706,15 +709,19 @@ __global__ __launch_bounds__(NT, 5) void k_stage32(GeomPtrs g, const PhysParams*
 #pragma unroll
             for (int k = 0; k < DIM; ++k) {
                 nan_flux |= __builtin_isnan(Fh[k]);
-                const double y = double(Fh[k]) - c[k];
-                const double s = sum[k] + y;
-                c[k] = (s - sum[k]) - y;
-                sum[k] = s;
+                if constexpr (STAGE == 1) {
+                    sum[k] += double(Fh[k]);
+                } else {
+                    const double y = double(Fh[k]) - c[k];
+                    const double s = sum[k] + y;
+                    c[k] = (s - sum[k]) - y;
+                    sum[k] = s;
+                }
             }
         }
         if (nan_flux) raise_stop(sc, splash::kStopNaNFlux, i);
         // stage 1's result only feeds the FP32 input of stage 2: the
-        // increments leave the compensated FP64 sums rounded to FP32 and are
+        // increments leave the FP64 sums rounded to FP32 and are
         // added to the face's own FP32 stage input (LDS entry t, the p-form
         // of U0); the pressure moves by (gam-1) (dE - dK), K = |r x L|^2 /
         // (2 Sigma), so no E-form state is formed.  That difference cancels
