@@ -380,8 +380,9 @@ def test_hex9_fp64_parity_with_split_dt_reduction():
     step's partials over 15 workgroups and the last one finishes dt.  With
     dt0 = 1e-3 the CFL clamp is active, so dt is exactly that reduction's
     result: after 2 steps the FP64 state matches the oracle to 1e-10 and dt
-    and time to round-off; the FP32-flux path's first dt (from its own
-    prepare kernel's partials) agrees too."""
+    and time to round-off; the FP32-flux path's first dt (its stage 1's
+    reduction of the limits its prepare kernel folded into the accumulator
+    slots) agrees too."""
     import splash_amd as S
     mesh = S.icosphere_hex(9)
     U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=8, omega=(0.1, 0.0, 0.3))
