@@ -154,19 +154,22 @@ def test_graph_replay_equals_eager(precision):
     """Graph replays equal eager steps; for the mixed path the replays also
     carry its dt (formed in stage 1 from the accumulator buffer the previous
     stage 2 filled, the buffer chosen from the device step count), so any
-    mix of replays and eager steps must keep the buffers in step."""
+    mix of replays and eager steps must keep the buffers in step.  dt0
+    changes between calls (the replays read it from a device scalar): 1.0
+    leaves dt to the CFL clamp, 5e-5 sets it."""
     import splash_amd as S
     mesh = S.icosphere_hex(4)
     U0 = S.ic.adiabatic_state(mesh.face_centres(), GAM, seed=5, omega=(0, 0, 0.2))
     prec = getattr(S, precision)
     a = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
     b = S.MUSCL_HLLCplus(mesh, U0, 5, GAM, 0.0, precision=prec)
-    for _ in range(3):
-        a.do_steps(1.0, 19)  # two 8-step graph replays + 3 eager steps each time; dt clamped
-    for _ in range(57):
-        b.do_step(1.0)
-    assert np.array_equal(a.get_state(), b.get_state())
-    assert a.time() == b.time() and a.last_dt() < 1e-2
+    for dt0 in (1.0, 5e-5, 1.0, 5e-5):
+        a.do_steps(dt0, 19)  # two 8-step graph replays + 3 eager steps each time
+        for _ in range(19):
+            b.do_step(dt0)
+        assert np.array_equal(a.get_state(), b.get_state())
+        assert a.time() == b.time() and a.last_dt() == b.last_dt()
+        assert (a.last_dt() < 1e-2) if dt0 == 1.0 else (a.last_dt() == dt0)
 
 
 def test_mixed_precision_error_and_conservation():
