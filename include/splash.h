@@ -238,7 +238,9 @@ void splash_geometry_free(splash_geometry *g);
  * time and stop_check and evolves exactly as splash_create(...) with
  * params.precision = SPLASH_MIXED would on the same inputs.
  * params[r].precision is ignored; params[0].setup_threads sets the host
- * threads of the geometry precompute.  U_in[r] is [n_faces(r)][dim]. */
+ * threads of the geometry precompute.  U_in[r] is [n_faces(r)][dim].  At
+ * most 65536 runs; each run's dt accumulators take 128 KiB of device
+ * memory beside its tables and state. */
 typedef struct splash_ensemble splash_ensemble;
 int splash_ensemble_create(int n_runs, const splash_mesh_desc *meshes, const double *const *U_in, int dim,
                            int solver, const double *gam, const double *omega_ns, const splash_params *params,

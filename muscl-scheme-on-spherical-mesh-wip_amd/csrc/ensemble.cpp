@@ -32,6 +32,7 @@ Ensemble::Ensemble(const std::vector<Run>& runs, int dim, int solver, int geomet
                    int block_threads)
     : dim_(dim), solver_(solver) {
     if (runs.empty()) throw std::invalid_argument("an ensemble needs at least one run");
+    if (runs.size() > 65536) throw std::invalid_argument("an ensemble holds at most 65536 runs");
     if (patch_faces < -1 || patch_faces > kPatchFacesMax) throw std::invalid_argument("patch_faces out of range");
     if (block_threads != 0 && !mixed::valid_threads(block_threads))
         throw std::invalid_argument("block_threads must be 0, 128, 192 or 256");
